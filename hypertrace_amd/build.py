@@ -1,0 +1,77 @@
+"""Build steps: the HIP runtime library (hipcc, gfx950) and the ahead-of-time
+compiled render programs of the standard workloads (hipRTC into the
+code-object cache, so a fresh GPU box does not compile them again)."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import time
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+HIPCC = "/opt/rocm/bin/hipcc"
+LIB = os.path.join(PKG, "lib", "libhypertrace_hip.so")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # No packed FP32 instructions in any HIP code of this project.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def run(cmd, **kw):
+    r = subprocess.run(cmd, capture_output=True, text=True, **kw)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}")
+    return r
+
+
+def build_library() -> str:
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    tmp = LIB + ".tmp"
+    run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(PKG, "csrc", "runtime.hip"), "-L/opt/rocm/lib", "-lhiprtc",
+         "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def standard_scenes():
+    """Scene instances whose programs are compiled ahead of time (the program
+    depends on the scene type only)."""
+    from . import scenes
+
+    return [
+        scenes.euclidean_spheres().scene,
+        scenes.euclidean_spheres(unit_spheres=True).scene,
+        scenes.hyperbolic_cells().scene,
+    ]
+
+
+def precompile_programs(verbose: bool = False) -> None:
+    from .processing import precompile
+    from .types import X32_F32, X32_F64
+
+    for scene in standard_scenes():
+        for cfg in (X32_F32, X32_F64):
+            t = time.time()
+            hit = precompile(scene, cfg)
+            if verbose:
+                print(f"  program {scene.geo} f64={cfg.double_support}: {'cached' if hit else 'compiled'} "
+                      f"in {time.time() - t:.1f}s")
+
+
+def build_all(verbose: bool = True) -> None:
+    t = time.time()
+    build_library()
+    if verbose:
+        print(f"built {LIB} in {time.time() - t:.1f}s")
+    precompile_programs(verbose)
+
+
+if __name__ == "__main__":
+    build_all()

@@ -1,0 +1,76 @@
+"""Scene-to-kernel codegen: the root source and header map of one render
+program.
+
+The reference built an OpenCL program per scene type from `kernel::SOURCE`
+plus generated headers and a root file that typedef'd the scene and included
+render/render.cl (processing/src/render.rs:14-54). Here the root source
+typedefs the scene's C++ template type, checks its byte layout against the
+host serializer with static_asserts, and instantiates the render kernels;
+the header map carries the device library.
+"""
+from __future__ import annotations
+
+import glob
+import os
+from typing import Dict, List, Tuple
+
+from .objects import Entity, SceneImpl, _ChoiceBase, _Component, _VecBase, Mapped
+from .types import Config
+
+DEVICE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "device")
+
+
+def device_headers() -> Dict[str, str]:
+    """Device library files keyed by include name ("ht/base.hh", ...)."""
+    out = {}
+    for path in sorted(glob.glob(os.path.join(DEVICE_DIR, "ht", "*.hh"))):
+        name = os.path.relpath(path, DEVICE_DIR)
+        with open(path) as f:
+            out[name] = f.read()
+    return out
+
+
+def _children(e: Entity) -> List[Entity]:
+    if isinstance(e, SceneImpl):
+        return [e.view, e.background, e.object]
+    if isinstance(e, _VecBase):
+        return [e.proto]
+    if isinstance(e, _ChoiceBase):
+        return list(e.variants)
+    if isinstance(e, Mapped):
+        return [e.map, e.inner]
+    kids = []
+    for name in ("inner", "shape", "material"):
+        if hasattr(e, name) and isinstance(getattr(e, name), Entity):
+            kids.append(getattr(e, name))
+    if hasattr(e, "components"):
+        kids += [c.material for c in e.components]
+    return kids
+
+
+def layout_asserts(scene: SceneImpl, cfg: Config) -> List[str]:
+    """static_asserts pinning every entity type's SIZE/ALIGN to the host
+    serializer's numbers."""
+    seen: Dict[str, Tuple[int, int]] = {}
+    stack: List[Entity] = [scene]
+    while stack:
+        e = stack.pop()
+        t = e.cpp(scene.geo)
+        if t not in seen:
+            seen[t] = (e.static_size(cfg), e.align(cfg))
+        stack.extend(_children(e))
+    lines = []
+    for t, (size, align) in sorted(seen.items()):
+        lines.append(f"static_assert({t}::SIZE == {size} && {t}::ALIGN == {align}, \"layout of {t}\");")
+    return lines
+
+
+def root_source(scene: SceneImpl, cfg: Config) -> str:
+    lines = [
+        "// Generated render program for one scene type.",
+        "#include <ht/render.hh>",
+        f"typedef {scene.cpp()} Scene;",
+    ]
+    lines += layout_asserts(scene, cfg)
+    lines.append("HT_DEFINE_RENDER_KERNELS(Scene)")
+    return "\n".join(lines) + "\n"

@@ -1,0 +1,135 @@
+// Render kernels for one scene type.
+//
+// render       one pass = one path per pixel, accumulated into the float4
+//              canvas; the drop-in for the reference kernel
+//              `render(uint2 shape, Scene*, float4* canvas, uint* seeds)`
+//              (kernel/ocl/render/render.cl:6-27).
+// render_passes  `passes` consecutive passes in one launch. Each thread owns a
+//              pixel and, when its path ends, starts the pixel's next sample
+//              in place (path regeneration), so the wavefront keeps running the
+//              wavefront-uniform nearest-hit scan on full lanes. Per pixel the
+//              sample sequence, the RNG stream and the order of the canvas
+//              additions are the same as `passes` calls of `render`, so the
+//              canvas and seeds come out identical.
+#pragma once
+#include "entities.hh"
+
+namespace ht {
+
+struct PixelSetup {
+    real2 pos;
+    real2 size;
+};
+
+// render.cl:12-18
+HD PixelSetup pixel_setup(uint x, uint y, uint w, uint h) {
+    PixelSetup p;
+    p.pos = r2((real)(2 * x + 1) - (real)w, (real)h - (real)(2 * y + 1)) / (real)h;
+    real s = (real)2 / (real)h;
+    p.size = r2(s, s);
+    return p;
+}
+
+template <class S>
+HD void render_pass_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
+                         uint *__restrict__ seeds) {
+    uint idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= w * h) return;
+    uint x = idx % w, y = idx / w;
+    PixelSetup px = pixel_setup(x, y, w, h);
+    Context ctx;
+    ctx.rng.state = seeds[idx];
+    ctx.hasher = hash_init();
+    ctx.prev_hash = hash_never();
+    color3 c = S::sample(scene, &ctx, px.pos, px.size);
+    float4 acc = canvas[idx];
+    acc.x += c.x;
+    acc.y += c.y;
+    acc.z += c.z;
+    acc.w += 1.0f;
+    canvas[idx] = acc;
+    seeds[idx] = ctx.rng.state;
+}
+
+template <class S>
+HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
+                           uint *__restrict__ seeds, uint passes) {
+    typedef typename S::Geo G;
+    typedef typename S::Object T;
+    constexpr int H = S::HOPS;
+    uint idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= w * h) return;
+    uint x = idx % w, y = idx / w;
+    PixelSetup px = pixel_setup(x, y, w, h);
+    Context ctx;
+    ctx.rng.state = seeds[idx];
+    ctx.hasher = hash_init();
+    ctx.prev_hash = hash_never();
+    float4 acc = canvas[idx];
+    const u8 *obj = S::object(scene);
+
+    if constexpr (H <= 0) {
+        for (uint k = 0; k < passes; ++k) {
+            (void)S::sample(scene, &ctx, px.pos, px.size);
+            acc.w += 1.0f;
+        }
+    } else {
+        Light<G> light;
+        color3 emission = c3(0.0f);
+        int hop = 0;
+        uint done = 0;
+        bool start = true;
+        for (;;) {
+            if (start) {
+                if (done == passes) break;
+                light.base.intensity = c3(1.0f);
+                light.base.diffuse = false;
+                light.ray = S::View::sample(S::view(scene), &ctx, px.pos, px.size);
+                emission = c3(0.0f);
+                ctx.prev_hash = hash_never();
+                hop = 0;
+                start = false;
+            }
+            ctx.hasher = hash_init();
+            typename T::Cache cache;
+            real dist = T::detect(obj, &ctx, &cache, &light);
+            bool end;
+            if (dist > (real)-0.5f) {
+                bool ret = T::interact(obj, &ctx, &cache, &light, &emission);
+                if (ret) ctx.prev_hash = hash_finish(&ctx.hasher);
+                end = !ret;
+            } else {
+                S::bg(scene, &ctx, &light, &emission);
+                end = true;
+            }
+            ++hop;
+            if (hop >= H) end = true;
+            if (end) {
+                acc.x += emission.x;
+                acc.y += emission.y;
+                acc.z += emission.z;
+                acc.w += 1.0f;
+                ++done;
+                start = true;
+            }
+        }
+    }
+    canvas[idx] = acc;
+    seeds[idx] = ctx.rng.state;
+}
+
+}  // namespace ht
+
+// Kernel entry points for the scene type `S`, with C linkage so the runtime
+// finds them by name in the compiled code object.
+#define HT_DEFINE_RENDER_KERNELS(S)                                                                   \
+    extern "C" __global__ void __launch_bounds__(256) render(                                         \
+        unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas, \
+        unsigned *__restrict__ seeds) {                                                               \
+        ht::render_pass_impl<S>(w, h, scene, canvas, seeds);                                          \
+    }                                                                                                 \
+    extern "C" __global__ void __launch_bounds__(256) render_passes(                                  \
+        unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas, \
+        unsigned *__restrict__ seeds, unsigned passes) {                                              \
+        ht::render_passes_impl<S>(w, h, scene, canvas, seeds, passes);                                \
+    }

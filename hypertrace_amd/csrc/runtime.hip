@@ -1,0 +1,579 @@
+// C-ABI runtime of the render pass on one MI355X (gfx950): contexts, device
+// buffers, hipRTC program build with an on-disk code-object cache, render
+// launches, the post passes and the seed buffer. Declarations and the
+// reference code each entry point replaces are in include/hypertrace_hip.h.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cerrno>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/hypertrace_hip.h"
+
+namespace {
+
+thread_local std::string g_err = "";
+thread_local int g_cache_hit = 0;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            return fail(HT_ERR_BACKEND, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+        }                                                                                               \
+    } while (0)
+
+const char *kArch = "gfx950";
+const char *kCacheTag = "ht-codeobj-v1";
+
+uint64_t fnv1a(uint64_t h, const void *data, size_t n) {
+    const unsigned char *p = static_cast<const unsigned char *>(data);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+uint64_t fnv1a_str(uint64_t h, const std::string &s) {
+    uint64_t n = s.size();
+    h = fnv1a(h, &n, sizeof(n));
+    return fnv1a(h, s.data(), s.size());
+}
+
+std::string cache_dir() {
+    const char *env = std::getenv("HT_CACHE_DIR");
+    if (env && *env) return env;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&ht_last_error), &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        size_t slash = p.rfind('/');
+        std::string dir = slash == std::string::npos ? "." : p.substr(0, slash);
+        return dir + "/../_cache";
+    }
+    return "./_cache";
+}
+
+bool read_file(const std::string &path, std::vector<char> *out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.seekg(0, std::ios::end);
+    std::streamoff n = f.tellg();
+    if (n <= 0) return false;
+    f.seekg(0, std::ios::beg);
+    out->resize(static_cast<size_t>(n));
+    f.read(out->data(), n);
+    return static_cast<bool>(f);
+}
+
+void write_file_atomic(const std::string &dir, const std::string &path, const std::vector<char> &data) {
+    mkdir(dir.c_str(), 0755);
+    std::string tmp = path + ".tmp" + std::to_string(getpid());
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f) return;
+        f.write(data.data(), static_cast<std::streamsize>(data.size()));
+        if (!f) return;
+    }
+    std::rename(tmp.c_str(), path.c_str());
+}
+
+}  // namespace
+
+struct ht_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+struct ht_prog {
+    ht_ctx *ctx = nullptr;
+    hipModule_t module = nullptr;
+    hipFunction_t render = nullptr;
+    hipFunction_t render_passes = nullptr;
+};
+
+struct ht_buf {
+    ht_ctx *ctx = nullptr;
+    void *ptr = nullptr;
+    size_t size = 0;
+};
+
+// ------------------------------------------------------------------ post kernels
+
+// image = canvas / (float)passes, processing/src/canvas.rs:126-135.
+__global__ void __launch_bounds__(256) k_extract(uint32_t n, float inv_unused, uint32_t passes,
+                                                 const float4 *__restrict__ canvas, float4 *__restrict__ image) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p = (float)passes;
+    float4 c = canvas[i];
+    image[i] = make_float4(c.x / p, c.y / p, c.z / p, c.w / p);
+}
+
+__device__ __forceinline__ float gamma1(float x, float g) { return powf(fmaxf(x, 0.0f), g); }
+
+// out = pow(max(in, 0), gamma), processing/src/filter/gamma.rs:10-19.
+__global__ void __launch_bounds__(256) k_gamma(uint32_t n, float g, const float4 *__restrict__ in,
+                                               float4 *__restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 c = in[i];
+    out[i] = make_float4(gamma1(c.x, g), gamma1(c.y, g), gamma1(c.z, g), gamma1(c.w, g));
+}
+
+// (uchar)(255 * clamp(x, 0, 1)), conversion toward zero; clamp = fmin(fmax(x,0),1)
+// so NaN packs to 0. processing/src/canvas.rs:176-186.
+__device__ __forceinline__ uint32_t pack1(float x) {
+    return (uint32_t)(255.0f * fminf(fmaxf(x, 0.0f), 1.0f));
+}
+__device__ __forceinline__ uint32_t pack4(float4 c) {
+    return pack1(c.x) | (pack1(c.y) << 8) | (pack1(c.z) << 16) | (pack1(c.w) << 24);
+}
+
+__global__ void __launch_bounds__(256) k_pack(uint32_t n, const float4 *__restrict__ image,
+                                              uint32_t *__restrict__ rgba8) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rgba8[i] = pack4(image[i]);
+}
+
+// extract -> gamma -> pack in one pass: 16 B read + 4 B written per pixel.
+__global__ void __launch_bounds__(256) k_post(uint32_t n, uint32_t passes, float g,
+                                              const float4 *__restrict__ canvas, uint32_t *__restrict__ rgba8) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float p = (float)passes;
+    float4 c = canvas[i];
+    float4 e = make_float4(c.x / p, c.y / p, c.z / p, c.w / p);
+    float4 v = make_float4(gamma1(e.x, g), gamma1(e.y, g), gamma1(e.z, g), gamma1(e.w, g));
+    rgba8[i] = pack4(v);
+}
+
+// ------------------------------------------------------------------ seeds
+
+namespace {
+
+// SplitMix64 step as used by rand 0.8's Xoshiro256PlusPlus::seed_from_u64.
+uint64_t splitmix64(uint64_t *state) {
+    *state += 0x9e3779b97f4a7c15ull;
+    uint64_t z = *state;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+struct Xoshiro256pp {
+    uint64_t s[4];
+    uint64_t next() {
+        uint64_t result = rotl(s[0] + s[3], 23) + s[0];
+        uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return result;
+    }
+};
+
+}  // namespace
+
+extern "C" uint32_t ht_seed_hash(uint64_t seed, uint32_t index) {
+    // murmur3 fmix64 of (seed + golden * (index + 1)), high word.
+    uint64_t z = seed + 0x9e3779b97f4a7c15ull * ((uint64_t)index + 1);
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    z ^= z >> 33;
+    return (uint32_t)(z >> 32);
+}
+
+extern "C" int ht_seed_fill(uint32_t *dst, size_t n, int mode, uint64_t seed) {
+    if (!dst && n) return fail(HT_ERR_OTHER, "ht_seed_fill: null destination");
+    if (mode == 0) {
+        Xoshiro256pp rng;
+        uint64_t st = seed;
+        for (int i = 0; i < 4; ++i) rng.s[i] = splitmix64(&st);
+        if (!(rng.s[0] | rng.s[1] | rng.s[2] | rng.s[3])) {
+            st = 0;
+            for (int i = 0; i < 4; ++i) rng.s[i] = splitmix64(&st);
+        }
+        // Rng::fill on [u32] fills the byte view: 8 bytes per next_u64
+        // (little-endian: low word first); a trailing 4-byte remainder takes
+        // next_u32 = the high word of next_u64.
+        size_t i = 0;
+        for (; i + 2 <= n; i += 2) {
+            uint64_t v = rng.next();
+            dst[i] = (uint32_t)v;
+            dst[i + 1] = (uint32_t)(v >> 32);
+        }
+        if (i < n) dst[i] = (uint32_t)(rng.next() >> 32);
+        return HT_OK;
+    }
+    if (mode == 1) {
+        for (size_t i = 0; i < n; ++i) dst[i] = ht_seed_hash(seed, (uint32_t)i);
+        return HT_OK;
+    }
+    return fail(HT_ERR_OTHER, "ht_seed_fill: unknown mode " + std::to_string(mode));
+}
+
+// ------------------------------------------------------------------ API
+
+extern "C" const char *ht_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ht_ctx_create(int device_index, ht_ctx **out) {
+    if (!out) return fail(HT_ERR_OTHER, "ht_ctx_create: null out");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device_index < 0 || device_index >= count)
+        return fail(HT_ERR_OTHER, "ht_ctx_create: device index " + std::to_string(device_index) + " out of range (" +
+                                      std::to_string(count) + " devices)");
+    HIP_TRY(hipSetDevice(device_index));
+    ht_ctx *c = new ht_ctx();
+    c->device = device_index;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(HT_ERR_BACKEND, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return fail(HT_ERR_BACKEND, "hipEventCreate failed");
+    }
+    *out = c;
+    return HT_OK;
+}
+
+extern "C" void ht_ctx_destroy(ht_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    hipEventDestroy(ctx->t0);
+    hipEventDestroy(ctx->t1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" int ht_ctx_sync(ht_ctx *ctx) {
+    if (!ctx) return fail(HT_ERR_OTHER, "ht_ctx_sync: null ctx");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return HT_OK;
+}
+
+extern "C" int ht_prog_cache_hit(void) { return g_cache_hit; }
+
+namespace {
+
+// Compiles the program with hipRTC, or loads its code object from the cache.
+int get_code_object(const char *root_name, const char *root_src, int n_headers, const char *const *names,
+                    const char *const *contents, int double_precision, int address_width, std::vector<char> *code) {
+    g_cache_hit = 0;
+    if (!root_src || n_headers < 0 || (n_headers > 0 && (!names || !contents)))
+        return fail(HT_ERR_OTHER, "program build: invalid argument");
+    if (address_width != 32 && address_width != 64)
+        return fail(HT_ERR_OTHER, "program build: address_width must be 32 or 64");
+    std::string name = root_name ? root_name : "render.hip";
+    std::vector<std::string> opts = {
+        std::string("--offload-arch=") + kArch,
+        "-O3",
+        "-std=c++17",
+        "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-DHT_ADDRESS_WIDTH=" + std::to_string(address_width),
+    };
+    if (double_precision) opts.push_back("-DHT_DOUBLE=1");
+
+    if (const char *dump = std::getenv("HT_DUMP_SOURCE")) {
+        if (*dump) {
+            std::ofstream f("render.hip");
+            f << root_src;
+        }
+    }
+
+    uint64_t h = 1469598103934665603ull;
+    h = fnv1a_str(h, kCacheTag);
+    for (const auto &o : opts) h = fnv1a_str(h, o);
+    h = fnv1a_str(h, name);
+    h = fnv1a_str(h, root_src);
+    for (int i = 0; i < n_headers; ++i) {
+        h = fnv1a_str(h, names[i] ? names[i] : "");
+        h = fnv1a_str(h, contents[i] ? contents[i] : "");
+    }
+    char key[32];
+    std::snprintf(key, sizeof key, "%016llx", (unsigned long long)h);
+    std::string dir = cache_dir();
+    std::string path = dir + "/" + key + "." + kArch + ".hsaco";
+
+    if (read_file(path, code)) {
+        g_cache_hit = 1;
+        return HT_OK;
+    }
+    hiprtcProgram prog;
+    hiprtcResult r = hiprtcCreateProgram(&prog, root_src, name.c_str(), n_headers, contents, names);
+    if (r != HIPRTC_SUCCESS) return fail(HT_ERR_BACKEND, std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r));
+    std::vector<const char *> copts;
+    for (const auto &o : opts) copts.push_back(o.c_str());
+    r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    std::string log(log_size, '\0');
+    if (log_size) hiprtcGetProgramLog(prog, &log[0]);
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail(HT_ERR_BACKEND, std::string("hipRTC build of ") + name + " failed: " + hiprtcGetErrorString(r) + "\n" + log);
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(prog, &code_size);
+    code->resize(code_size);
+    hiprtcGetCode(prog, code->data());
+    hiprtcDestroyProgram(&prog);
+    write_file_atomic(dir, path, *code);
+    return HT_OK;
+}
+
+}  // namespace
+
+extern "C" int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
+                             const char *const *names, const char *const *contents, int double_precision,
+                             int address_width, ht_prog **out) {
+    if (!ctx || !out) return fail(HT_ERR_OTHER, "ht_prog_build: invalid argument");
+    std::vector<char> code;
+    int rc = get_code_object(root_name, root_src, n_headers, names, contents, double_precision, address_width, &code);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    ht_prog *p = new ht_prog();
+    p->ctx = ctx;
+    hipError_t e = hipModuleLoadData(&p->module, code.data());
+    if (e != hipSuccess) {
+        delete p;
+        return fail(HT_ERR_BACKEND, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
+    }
+    if (hipModuleGetFunction(&p->render, p->module, "render") != hipSuccess ||
+        hipModuleGetFunction(&p->render_passes, p->module, "render_passes") != hipSuccess) {
+        hipModuleUnload(p->module);
+        delete p;
+        return fail(HT_ERR_BACKEND, "program lacks the render/render_passes kernels");
+    }
+    *out = p;
+    return HT_OK;
+}
+
+extern "C" int ht_prog_precompile(const char *root_name, const char *root_src, int n_headers,
+                                  const char *const *names, const char *const *contents, int double_precision,
+                                  int address_width) {
+    std::vector<char> code;
+    return get_code_object(root_name, root_src, n_headers, names, contents, double_precision, address_width, &code);
+}
+
+extern "C" void ht_prog_destroy(ht_prog *prog) {
+    if (!prog) return;
+    hipSetDevice(prog->ctx->device);
+    hipModuleUnload(prog->module);
+    delete prog;
+}
+
+extern "C" int ht_buf_alloc(ht_ctx *ctx, size_t bytes, ht_buf **out) {
+    if (!ctx || !out) return fail(HT_ERR_OTHER, "ht_buf_alloc: invalid argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ht_buf *b = new ht_buf();
+    b->ctx = ctx;
+    b->size = bytes;
+    if (bytes) {
+        hipError_t e = hipMalloc(&b->ptr, bytes);
+        if (e != hipSuccess) {
+            delete b;
+            return fail(HT_ERR_BACKEND, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+    }
+    *out = b;
+    return HT_OK;
+}
+
+extern "C" int ht_buf_free(ht_buf *buf) {
+    if (!buf) return HT_OK;
+    HIP_TRY(hipSetDevice(buf->ctx->device));
+    HIP_TRY(hipStreamSynchronize(buf->ctx->stream));
+    if (buf->ptr) HIP_TRY(hipFree(buf->ptr));
+    delete buf;
+    return HT_OK;
+}
+
+extern "C" size_t ht_buf_size(const ht_buf *buf) { return buf ? buf->size : 0; }
+
+extern "C" int ht_buf_write(ht_buf *buf, const void *src, size_t bytes) {
+    if (!buf || (!src && bytes)) return fail(HT_ERR_OTHER, "ht_buf_write: invalid argument");
+    if (bytes > buf->size) return fail(HT_ERR_OTHER, "ht_buf_write: " + std::to_string(bytes) + " bytes into a buffer of " + std::to_string(buf->size));
+    if (!bytes) return HT_OK;
+    HIP_TRY(hipSetDevice(buf->ctx->device));
+    HIP_TRY(hipMemcpyAsync(buf->ptr, src, bytes, hipMemcpyHostToDevice, buf->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(buf->ctx->stream));
+    return HT_OK;
+}
+
+extern "C" int ht_buf_read(ht_buf *buf, void *dst, size_t bytes) {
+    if (!buf || (!dst && bytes)) return fail(HT_ERR_OTHER, "ht_buf_read: invalid argument");
+    if (bytes > buf->size) return fail(HT_ERR_OTHER, "ht_buf_read: " + std::to_string(bytes) + " bytes from a buffer of " + std::to_string(buf->size));
+    if (!bytes) return HT_OK;
+    HIP_TRY(hipSetDevice(buf->ctx->device));
+    HIP_TRY(hipMemcpyAsync(dst, buf->ptr, bytes, hipMemcpyDeviceToHost, buf->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(buf->ctx->stream));
+    return HT_OK;
+}
+
+extern "C" int ht_buf_zero(ht_buf *buf) {
+    if (!buf) return fail(HT_ERR_OTHER, "ht_buf_zero: null buffer");
+    if (!buf->size) return HT_OK;
+    HIP_TRY(hipSetDevice(buf->ctx->device));
+    HIP_TRY(hipMemsetAsync(buf->ptr, 0, buf->size, buf->ctx->stream));
+    return HT_OK;
+}
+
+namespace {
+
+int check_image(const char *what, uint32_t w, uint32_t h, const ht_buf *b, size_t px_bytes) {
+    if (!b) return fail(HT_ERR_OTHER, std::string(what) + ": null buffer");
+    size_t need = (size_t)w * (size_t)h * px_bytes;
+    if (b->size < need)
+        return fail(HT_ERR_OTHER, std::string(what) + ": buffer of " + std::to_string(b->size) + " bytes, need " + std::to_string(need));
+    return HT_OK;
+}
+
+int launch_render(ht_prog *prog, hipFunction_t fn, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas,
+                  ht_buf *seeds, uint32_t passes, bool with_passes) {
+    if (!prog || !scene) return fail(HT_ERR_OTHER, "render: invalid argument");
+    if (w == 0 || h == 0) return HT_OK;
+    if ((uint64_t)w * (uint64_t)h > 0xFFFFFFFFull) return fail(HT_ERR_OTHER, "render: image too large");
+    int rc;
+    if ((rc = check_image("render canvas", w, h, canvas, 16))) return rc;
+    if ((rc = check_image("render seeds", w, h, seeds, 4))) return rc;
+    if (scene->size == 0) return fail(HT_ERR_OTHER, "render: empty scene buffer");
+    ht_ctx *ctx = prog->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint32_t n = w * h;
+    uint32_t block = 256;
+    uint32_t grid = (n + block - 1) / block;
+    void *sp = scene->ptr, *cp = canvas->ptr, *pp = seeds->ptr;
+    void *args_pass[] = {&w, &h, &sp, &cp, &pp};
+    void *args_passes[] = {&w, &h, &sp, &cp, &pp, &passes};
+    HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, ctx->stream,
+                                  with_passes ? args_passes : args_pass, nullptr));
+    return HT_OK;
+}
+
+}  // namespace
+
+extern "C" int ht_render(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas, ht_buf *seeds) {
+    int rc = launch_render(prog, prog ? prog->render : nullptr, w, h, scene, canvas, seeds, 1, false);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(prog->ctx->stream));
+    return HT_OK;
+}
+
+extern "C" int ht_render_passes(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas, ht_buf *seeds,
+                                uint32_t passes) {
+    if (passes == 0) return HT_OK;
+    return launch_render(prog, prog ? prog->render_passes : nullptr, w, h, scene, canvas, seeds, passes, true);
+}
+
+extern "C" int ht_extract(ht_ctx *ctx, uint32_t w, uint32_t h, uint32_t passes, ht_buf *canvas, ht_buf *image) {
+    if (!ctx) return fail(HT_ERR_OTHER, "ht_extract: null ctx");
+    int rc;
+    if ((rc = check_image("ht_extract canvas", w, h, canvas, 16))) return rc;
+    if ((rc = check_image("ht_extract image", w, h, image, 16))) return rc;
+    uint32_t n = w * h;
+    if (!n) return HT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_extract, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, 0.0f, passes,
+                       (const float4 *)canvas->ptr, (float4 *)image->ptr);
+    HIP_TRY(hipGetLastError());
+    return HT_OK;
+}
+
+extern "C" int ht_gamma(ht_ctx *ctx, uint32_t w, uint32_t h, float gamma, ht_buf *in, ht_buf *out) {
+    if (!ctx) return fail(HT_ERR_OTHER, "ht_gamma: null ctx");
+    int rc;
+    if ((rc = check_image("ht_gamma in", w, h, in, 16))) return rc;
+    if ((rc = check_image("ht_gamma out", w, h, out, 16))) return rc;
+    uint32_t n = w * h;
+    if (!n) return HT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_gamma, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, gamma,
+                       (const float4 *)in->ptr, (float4 *)out->ptr);
+    HIP_TRY(hipGetLastError());
+    return HT_OK;
+}
+
+extern "C" int ht_pack(ht_ctx *ctx, uint32_t w, uint32_t h, ht_buf *image, ht_buf *rgba8) {
+    if (!ctx) return fail(HT_ERR_OTHER, "ht_pack: null ctx");
+    int rc;
+    if ((rc = check_image("ht_pack image", w, h, image, 16))) return rc;
+    if ((rc = check_image("ht_pack rgba8", w, h, rgba8, 4))) return rc;
+    uint32_t n = w * h;
+    if (!n) return HT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, (const float4 *)image->ptr,
+                       (uint32_t *)rgba8->ptr);
+    HIP_TRY(hipGetLastError());
+    return HT_OK;
+}
+
+extern "C" int ht_post(ht_ctx *ctx, uint32_t w, uint32_t h, uint32_t passes, float gamma, ht_buf *canvas,
+                       ht_buf *rgba8) {
+    if (!ctx) return fail(HT_ERR_OTHER, "ht_post: null ctx");
+    int rc;
+    if ((rc = check_image("ht_post canvas", w, h, canvas, 16))) return rc;
+    if ((rc = check_image("ht_post rgba8", w, h, rgba8, 4))) return rc;
+    uint32_t n = w * h;
+    if (!n) return HT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, passes, gamma,
+                       (const float4 *)canvas->ptr, (uint32_t *)rgba8->ptr);
+    HIP_TRY(hipGetLastError());
+    return HT_OK;
+}
+
+extern "C" int ht_seed_init(ht_buf *seeds, uint32_t w, uint32_t h, int mode, uint64_t seed) {
+    int rc;
+    if ((rc = check_image("ht_seed_init", w, h, seeds, 4))) return rc;
+    std::vector<uint32_t> host((size_t)w * h);
+    if ((rc = ht_seed_fill(host.data(), host.size(), mode, seed))) return rc;
+    return ht_buf_write(seeds, host.data(), host.size() * 4);
+}
+
+extern "C" int ht_timer_begin(ht_ctx *ctx) {
+    if (!ctx) return fail(HT_ERR_OTHER, "ht_timer_begin: null ctx");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipEventRecord(ctx->t0, ctx->stream));
+    return HT_OK;
+}
+
+extern "C" int ht_timer_end(ht_ctx *ctx, float *ms) {
+    if (!ctx || !ms) return fail(HT_ERR_OTHER, "ht_timer_end: invalid argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipEventRecord(ctx->t1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->t1));
+    HIP_TRY(hipEventElapsedTime(ms, ctx->t0, ctx->t1));
+    return HT_OK;
+}

@@ -1,0 +1,119 @@
+/* hypertrace_hip.h — C ABI of the MI355X (gfx950) render pass.
+ *
+ * Drop-in for the OpenCL plumbing of the reference's `processing` crate:
+ * every entry point names the reference code it replaces. All functions
+ * return 0 on success and a negative code on failure; ht_last_error() then
+ * holds a message (thread-local). Handles are opaque and owned by the caller.
+ * One in-order HIP stream per context; calls on one context are not
+ * reentrant.
+ */
+#ifndef HYPERTRACE_HIP_H
+#define HYPERTRACE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ht_ctx ht_ctx;
+typedef struct ht_prog ht_prog;
+typedef struct ht_buf ht_buf;
+
+/* Error codes, mapped onto processing::Error (processing/src/error.rs:3-9). */
+#define HT_OK 0
+#define HT_ERR_IO (-1)      /* Error::Io */
+#define HT_ERR_BACKEND (-2) /* Error::Backend: HIP / hipRTC failure */
+#define HT_ERR_OTHER (-3)   /* Error::Other: bad argument, size mismatch */
+
+/* Last error message of the calling thread (never NULL). */
+const char *ht_last_error(void);
+
+/* Context: device + one in-order stream.
+ * Replaces main/src/cli.rs:62-71 (platform/device/queue selection) and
+ * processing/src/context.rs:3-16. */
+int ht_ctx_create(int device_index, ht_ctx **out);
+void ht_ctx_destroy(ht_ctx *ctx);
+/* Block until all work queued on the context's stream is done. */
+int ht_ctx_sync(ht_ctx *ctx);
+
+/* Program build with hipRTC from a root source plus an in-memory header map
+ * (names/contents), for one scene type. Replaces Render::source/new
+ * (processing/src/render.rs:14-100), which assembled the OpenCL program from
+ * kernel::SOURCE plus the generated scene headers and built it with
+ * ocl::Program. double_precision mirrors DOUBLE_PRECISION, address_width
+ * mirrors ADDRESS_WIDTH (32 or 64). Compiled code objects are cached on disk
+ * under $HT_CACHE_DIR (default: the library's directory + "/../_cache"),
+ * keyed by a hash of sources and options. If $HT_DUMP_SOURCE is set, the root
+ * source is written to ./render.hip (render.rs:74-80 wrote ./render.ocl). */
+int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
+                  const char *const *names, const char *const *contents, int double_precision,
+                  int address_width, ht_prog **out);
+void ht_prog_destroy(ht_prog *prog);
+/* 1 if the last build on this thread was served from the cache. */
+int ht_prog_cache_hit(void);
+/* Compile only, into the code-object cache; needs no GPU. No reference
+ * counterpart: lets a build step prepare programs ahead of time. */
+int ht_prog_precompile(const char *root_name, const char *root_src, int n_headers,
+                       const char *const *names, const char *const *contents, int double_precision,
+                       int address_width);
+
+/* Device buffers. Replace ocl::Buffer use in processing/src/buffer.rs:8-34
+ * (EntityBuffer) and canvas.rs:125-189 (ImageBuffer). Writes copy the host
+ * bytes; reads block until the data is on the host (canvas.rs:183-188). */
+int ht_buf_alloc(ht_ctx *ctx, size_t bytes, ht_buf **out);
+int ht_buf_free(ht_buf *buf);
+size_t ht_buf_size(const ht_buf *buf);
+int ht_buf_write(ht_buf *buf, const void *src, size_t bytes);
+int ht_buf_read(ht_buf *buf, void *dst, size_t bytes);
+int ht_buf_zero(ht_buf *buf);
+
+/* One pass: one path per pixel, canvas[idx] += (radiance, 1), seeds advanced.
+ * Synchronous like Render::render (processing/src/render.rs:102-118, the
+ * queue.finish() at :115). canvas: float4[w*h]; seeds: uint32[w*h]. */
+int ht_render(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas_f32x4,
+              ht_buf *seeds_u32);
+
+/* `passes` consecutive passes in one launch with in-thread path
+ * regeneration; canvas and seeds end up identical to `passes` calls of
+ * ht_render. No reference counterpart (the reference runs one launch + one
+ * finish per pass). Asynchronous: completes on the context's stream. */
+int ht_render_passes(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas_f32x4,
+                     ht_buf *seeds_u32, uint32_t passes);
+
+/* Post passes (asynchronous until a read). image, out: float4[w*h];
+ * rgba8: uchar4[w*h].
+ * ht_extract: image = canvas / (float)passes   (processing/src/canvas.rs:126-135)
+ * ht_gamma:   out = pow(max(in, 0), gamma)      (processing/src/filter/gamma.rs:10-19)
+ * ht_pack:    rgba8 = (uchar)(255 * clamp(x, 0, 1)) truncated (canvas.rs:176-186)
+ * ht_post:    the three fused in one kernel, bit-identical to running them
+ *             in order (Pipeline::get_image, processing/src/pipeline.rs:53-62). */
+int ht_extract(ht_ctx *ctx, uint32_t w, uint32_t h, uint32_t passes, ht_buf *canvas, ht_buf *image);
+int ht_gamma(ht_ctx *ctx, uint32_t w, uint32_t h, float gamma, ht_buf *in, ht_buf *out);
+int ht_pack(ht_ctx *ctx, uint32_t w, uint32_t h, ht_buf *image, ht_buf *rgba8);
+int ht_post(ht_ctx *ctx, uint32_t w, uint32_t h, uint32_t passes, float gamma, ht_buf *canvas,
+            ht_buf *rgba8);
+
+/* Per-pixel seed buffer. Replaces Canvas::new's seed fill
+ * (processing/src/canvas.rs:73-84).
+ * mode 0: rand 0.8 SmallRng::seed_from_u64(seed).fill(&mut [u32]) on 64-bit
+ *         targets (xoshiro256++ seeded by SplitMix64); the reference uses
+ *         seed = 0xdeadbeefdeadbeef.
+ * mode 1: seeds[i] = ht_seed_hash(seed, i) (per-pixel hash seeding). */
+int ht_seed_init(ht_buf *seeds, uint32_t w, uint32_t h, int mode, uint64_t seed);
+/* Host-side seed generation, same modes, into a caller array of n words. */
+int ht_seed_fill(uint32_t *dst, size_t n, int mode, uint64_t seed);
+uint32_t ht_seed_hash(uint64_t seed, uint32_t index);
+
+/* Timing on the context's stream with HIP events: ht_timer_begin records a
+ * start event; ht_timer_end records a stop event, waits for it and returns
+ * the elapsed milliseconds. */
+int ht_timer_begin(ht_ctx *ctx);
+int ht_timer_end(ht_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYPERTRACE_HIP_H */

@@ -1,0 +1,100 @@
+"""CPU oracle of the render pass — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+this package, and only as the checker. The product path never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_size_t, c_uint8, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+F32, F64, COUNT = 0, 1, 2
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run __graft_entry__.build())")
+        l = ctypes.CDLL(LIB_PATH)
+        l.oracle_last_error.restype = c_char_p
+        l.oracle_render.restype = c_int
+        l.oracle_render.argtypes = [c_char_p, c_int, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p, c_uint32,
+                                    c_uint32, c_uint32, c_uint32, c_int, POINTER(c_uint64)]
+        l.oracle_lcg_next.restype = c_uint32
+        l.oracle_lcg_next.argtypes = [POINTER(c_uint32)]
+        l.oracle_uniform_f32.restype = c_float
+        l.oracle_uniform_f32.argtypes = [POINTER(c_uint32)]
+        l.oracle_hemisphere_cosine_f64.argtypes = [POINTER(c_uint32), c_void_p]
+        l.oracle_hash.restype = c_uint32
+        l.oracle_hash.argtypes = [c_void_p, c_int]
+        for n in ("oracle_q_mul", "oracle_q_div", "oracle_c22_dot", "oracle_mo_apply", "oracle_rot_apply"):
+            getattr(l, n).argtypes = [c_void_p, c_void_p, c_void_p]
+        l.oracle_mo_deriv.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+        l.oracle_hy_distance.restype = c_double
+        l.oracle_hy_distance.argtypes = [c_void_p, c_void_p]
+        l.oracle_look_at.argtypes = [c_void_p, c_void_p]
+        l.oracle_shape_detect.restype = c_double
+        l.oracle_shape_detect.argtypes = [c_int, c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+        l.oracle_post.argtypes = [c_void_p, c_size_t, c_uint32, c_float, c_void_p, c_void_p]
+        _lib = l
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(c_void_p)
+
+
+def render(spec: str, w: int, h: int, passes: int, seeds: np.ndarray, canvas: np.ndarray = None,
+           precision: int = F32, crop=None, threads: int = 0):
+    """Returns (canvas float32[h, w, 4], seeds uint32[h, w], flops). Only the
+    crop (x0, y0, cw, ch) is rendered; other pixels keep their inputs."""
+    seeds = np.array(seeds, dtype=np.uint32).reshape(h, w).copy()
+    canvas = np.zeros((h, w, 4), np.float32) if canvas is None else np.array(canvas, np.float32).reshape(h, w, 4).copy()
+    x0, y0, cw, ch = crop if crop is not None else (0, 0, w, h)
+    flops = c_uint64(0)
+    rc = lib().oracle_render(spec.encode(), precision, w, h, passes, _p(canvas), _p(seeds), x0, y0, cw, ch, threads,
+                             ctypes.byref(flops))
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return canvas, seeds, int(flops.value)
+
+
+def vec_call(name: str, *arrays, out_len: int = 4):
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    out = np.zeros(out_len, np.float64)
+    getattr(lib(), name)(*[_p(a) for a in args], _p(out))
+    return out
+
+
+def hy_distance(a, b) -> float:
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return float(lib().oracle_hy_distance(_p(a), _p(b)))
+
+
+def shape_detect(kind: int, start, direction, repeat: bool = False, param: float = 0.0):
+    s = np.zeros(4, np.float64)
+    d = np.zeros(4, np.float64)
+    s[: len(start)] = start
+    d[: len(direction)] = direction
+    os_, od, on = np.zeros(4), np.zeros(4), np.zeros(4)
+    dist = lib().oracle_shape_detect(kind, param, int(repeat), _p(s), _p(d), _p(os_), _p(od), _p(on))
+    return float(dist), os_, od, on
+
+
+def post(canvas: np.ndarray, passes: int, gamma: float):
+    canvas = np.ascontiguousarray(canvas, np.float32)
+    n = canvas.size // 4
+    image = np.zeros(canvas.shape, np.float32)
+    rgba = np.zeros(canvas.shape, np.uint8)
+    lib().oracle_post(_p(canvas), n, passes, gamma, _p(image), _p(rgba))
+    return image, rgba
