@@ -77,7 +77,7 @@ def main():
         sys.exit(2)
 
     from hypertrace_amd import scenes
-    from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render, seed_fill
+    from hypertrace_amd.processing import FAST_FP, Canvas, Context, EntityBuffer, Render, seed_fill
     from hypertrace_amd.types import X32_F32
 
     wl = scenes.hyperbolic_cells()
@@ -87,26 +87,36 @@ def main():
     seeds0 = seed_fill(W * H, wl.seed_mode, wl.seed)
 
     with Context(0, X32_F32) as ctx:
-        render = Render(ctx, wl.scene)
         buf = EntityBuffer(ctx, wl.scene)
-        canvas = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
 
-        for _ in range(args.warmup):
-            render.render_passes(buf, canvas, SPP)
-        ctx.sync()
+        def time_chosen(render, steps, warmup):
+            canvas = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+            for _ in range(warmup):
+                render.render_passes(buf, canvas, SPP)
+            ctx.sync()
+            # timed region: K steps, HIP events on the launch stream
+            t0 = time.perf_counter()
+            ctx.timer_begin()
+            for _ in range(steps):
+                render.render_passes(buf, canvas, SPP)
+            gpu_ms = ctx.timer_end()
+            ctx.sync()
+            wall = time.perf_counter() - t0
+            return wall, gpu_ms
 
-        # timed region: K steps of the chosen form, HIP events on its stream
-        ctx.sync()
-        t0 = time.perf_counter()
-        ctx.timer_begin()
-        for _ in range(args.steps):
-            render.render_passes(buf, canvas, SPP)
-        gpu_ms = ctx.timer_end()
-        ctx.sync()
-        wall = time.perf_counter() - t0
+        render = Render(ctx, wl.scene)  # default program: strict IEEE FP32
+        wall, gpu_ms = time_chosen(render, args.steps, args.warmup)
         ms_per_step = wall * 1e3 / args.steps
         kernel_ms = gpu_ms / args.steps
         mpaths = paths_per_step * args.steps / wall / 1e6
+
+        fast = None
+        if not args.no_plain:
+            render_fast = Render(ctx, wl.scene, fp_options=FAST_FP)
+            fwall, fgpu = time_chosen(render_fast, args.steps, args.warmup)
+            fast = {"kernel": "render_passes, fast FP32 (FMA contraction, reciprocal division, device libm)",
+                    "value": paths_per_step * args.steps / fwall / 1e6, "unit": "Mpaths/s",
+                    "kernel_ms": fgpu / args.steps}
 
         plain = None
         if not args.no_plain:
@@ -119,8 +129,9 @@ def main():
                 render.render(buf, canvas_p)
             plain_gpu_ms = ctx.timer_end()
             plain_wall = time.perf_counter() - t0
-            plain = {"kernel": "render (one launch + sync per pass)", "value": W * H * SPP / plain_wall / 1e6,
-                     "unit": "Mpaths/s", "ms_per_frame": plain_wall * 1e3, "gpu_ms_per_frame": plain_gpu_ms}
+            plain = {"kernel": "render (one launch + sync per pass, as the reference runs)",
+                     "value": W * H * SPP / plain_wall / 1e6, "unit": "Mpaths/s",
+                     "ms_per_frame": plain_wall * 1e3, "gpu_ms_per_frame": plain_gpu_ms}
 
     flops_per_path, sample_paths = count_flops_per_path(spec, W, H, seeds0)
     flops_per_launch = flops_per_path * paths_per_step
@@ -148,6 +159,7 @@ def main():
                    "primitives": 192, "kernel": "render_passes"},
         "roofline": roofline,
         "plain": plain,
+        "fast_fp": fast,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, W, H, seeds0)

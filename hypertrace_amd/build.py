@@ -53,16 +53,16 @@ def standard_scenes():
 
 
 def precompile_programs(verbose: bool = False) -> None:
-    from .processing import precompile
+    from .processing import FAST_FP, precompile
     from .types import X32_F32, X32_F64
 
     for scene in standard_scenes():
-        for cfg in (X32_F32, X32_F64):
+        for cfg, fp in ((X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP)):
             t = time.time()
-            hit = precompile(scene, cfg)
+            hit = precompile(scene, cfg, fp_options=fp)
             if verbose:
-                print(f"  program {scene.geo} f64={cfg.double_support}: {'cached' if hit else 'compiled'} "
-                      f"in {time.time() - t:.1f}s")
+                print(f"  program {scene.geo} f64={cfg.double_support} fp={fp or 'strict'}: "
+                      f"{'cached' if hit else 'compiled'} in {time.time() - t:.1f}s")
 
 
 def build_all(verbose: bool = True) -> None:

@@ -143,26 +143,47 @@ HD real3 operator-(real3 a) { return {-a.x, -a.y, -a.z}; }
 HD real4 operator-(real4 a) { return {-a.x, -a.y, -a.z, -a.w}; }
 
 // ---------------------------------------------------------------- math
-// Geometry runs in `real`; colours in float. Division and square roots use the
-// hardware's fast forms in FP32 (the reference relies on OpenCL's relaxed
-// 2.5-ulp division); FP64 uses the IEEE forms.
+// Geometry runs in `real`; colours in float.
+//
+// Two FP32 code-generation modes, chosen at program build:
+//  HT_STRICT_FP (the default program build): every operation rounds on its
+//    own (no FMA contraction, IEEE division and square root, normalize as
+//    v / length(v)), and FP32 transcendentals are evaluated in FP64 and
+//    rounded once, which is correctly rounded except in ~2^-28 of cases. The
+//    result no longer depends on one libm's ulp errors, so it agrees with any
+//    other correctly rounded evaluation of the reference formulas.
+//  fast: FMA contraction, hardware reciprocal-based division (within the
+//    2.5 ulp OpenCL allows the reference), rsqrt-based normalize, the FP32
+//    device libm.
 HD float sqrt_(float x) { return __builtin_sqrtf(x); }
 HD double sqrt_(double x) { return __builtin_sqrt(x); }
 HD float rsqrt_(float x) { return __builtin_amdgcn_rsqf(x); }
 HD double rsqrt_(double x) { return 1.0 / __builtin_sqrt(x); }
+#ifdef HT_STRICT_FP
+HD float log_(float x) { return (float)::log((double)x); }
+HD float exp_(float x) { return (float)::exp((double)x); }
+HD float cos_(float x) { return (float)::cos((double)x); }
+HD float sin_(float x) { return (float)::sin((double)x); }
+HD float cosh_(float x) { return (float)::cosh((double)x); }
+HD float sinh_(float x) { return (float)::sinh((double)x); }
+HD float atan2_(float y, float x) { return (float)::atan2((double)y, (double)x); }
+HD float powf_(float a, float b) { return (float)::pow((double)a, (double)b); }
+#else
 HD float log_(float x) { return ::logf(x); }
-HD double log_(double x) { return ::log(x); }
 HD float exp_(float x) { return ::expf(x); }
-HD double exp_(double x) { return ::exp(x); }
 HD float cos_(float x) { return ::cosf(x); }
-HD double cos_(double x) { return ::cos(x); }
 HD float sin_(float x) { return ::sinf(x); }
-HD double sin_(double x) { return ::sin(x); }
 HD float cosh_(float x) { return ::coshf(x); }
-HD double cosh_(double x) { return ::cosh(x); }
 HD float sinh_(float x) { return ::sinhf(x); }
-HD double sinh_(double x) { return ::sinh(x); }
 HD float atan2_(float y, float x) { return ::atan2f(y, x); }
+HD float powf_(float a, float b) { return ::powf(a, b); }
+#endif
+HD double log_(double x) { return ::log(x); }
+HD double exp_(double x) { return ::exp(x); }
+HD double cos_(double x) { return ::cos(x); }
+HD double sin_(double x) { return ::sin(x); }
+HD double cosh_(double x) { return ::cosh(x); }
+HD double sinh_(double x) { return ::sinh(x); }
 HD double atan2_(double y, double x) { return ::atan2(y, x); }
 HD float fabs_(float x) { return __builtin_fabsf(x); }
 HD double fabs_(double x) { return __builtin_fabs(x); }
@@ -172,7 +193,6 @@ HD float fmax_(float a, float b) { return __builtin_fmaxf(a, b); }
 HD double fmax_(double a, double b) { return __builtin_fmax(a, b); }
 HD float floor_(float x) { return __builtin_floorf(x); }
 HD double floor_(double x) { return __builtin_floor(x); }
-HD float powf_(float a, float b) { return ::powf(a, b); }
 HD real sign_(real a) { return real((R0 < a) - (a < R0)); }
 
 HD real dot(real2 a, real2 b) { return a.x * b.x + a.y * b.y; }
@@ -180,7 +200,12 @@ HD real dot(real3 a, real3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 HD real dot(real4 a, real4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 template <class V> HD real length2(V a) { return dot(a, a); }
 template <class V> HD real length(V a) { return sqrt_(dot(a, a)); }
+#ifdef HT_STRICT_FP
+// OpenCL normalize(v) = v / length(v), each operation rounded on its own.
+template <class V> HD V normalize(V a) { return a / sqrt_(dot(a, a)); }
+#else
 template <class V> HD V normalize(V a) { return a * rsqrt_(dot(a, a)); }
+#endif
 HD real3 cross(real3 a, real3 b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }

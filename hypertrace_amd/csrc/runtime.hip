@@ -287,8 +287,14 @@ extern "C" int ht_prog_cache_hit(void) { return g_cache_hit; }
 namespace {
 
 // Compiles the program with hipRTC, or loads its code object from the cache.
+// Floating-point code-generation options of a program build when the caller
+// gives none: every operation rounded on its own (no FMA contraction, IEEE
+// division and square root) and the device library's strict mode.
+const char *kDefaultFpOptions = "-ffp-contract=off -DHT_STRICT_FP=1";
+
 int get_code_object(const char *root_name, const char *root_src, int n_headers, const char *const *names,
-                    const char *const *contents, int double_precision, int address_width, std::vector<char> *code) {
+                    const char *const *contents, int double_precision, int address_width, const char *fp_options,
+                    std::vector<char> *code) {
     g_cache_hit = 0;
     if (!root_src || n_headers < 0 || (n_headers > 0 && (!names || !contents)))
         return fail(HT_ERR_OTHER, "program build: invalid argument");
@@ -299,11 +305,15 @@ int get_code_object(const char *root_name, const char *root_src, int n_headers, 
         std::string("--offload-arch=") + kArch,
         "-O3",
         "-std=c++17",
-        "-fno-hip-fp32-correctly-rounded-divide-sqrt",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-DHT_ADDRESS_WIDTH=" + std::to_string(address_width),
     };
     if (double_precision) opts.push_back("-DHT_DOUBLE=1");
+    {
+        std::istringstream fp(fp_options ? fp_options : kDefaultFpOptions);
+        std::string o;
+        while (fp >> o) opts.push_back(o);
+    }
 
     if (const char *dump = std::getenv("HT_DUMP_SOURCE")) {
         if (*dump) {
@@ -355,12 +365,13 @@ int get_code_object(const char *root_name, const char *root_src, int n_headers, 
 
 }  // namespace
 
-extern "C" int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
-                             const char *const *names, const char *const *contents, int double_precision,
-                             int address_width, ht_prog **out) {
+extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
+                                const char *const *names, const char *const *contents, int double_precision,
+                                int address_width, const char *fp_options, ht_prog **out) {
     if (!ctx || !out) return fail(HT_ERR_OTHER, "ht_prog_build: invalid argument");
     std::vector<char> code;
-    int rc = get_code_object(root_name, root_src, n_headers, names, contents, double_precision, address_width, &code);
+    int rc = get_code_object(root_name, root_src, n_headers, names, contents, double_precision, address_width,
+                             fp_options, &code);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     ht_prog *p = new ht_prog();
@@ -380,11 +391,19 @@ extern "C" int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *roo
     return HT_OK;
 }
 
+extern "C" int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
+                             const char *const *names, const char *const *contents, int double_precision,
+                             int address_width, ht_prog **out) {
+    return ht_prog_build_ex(ctx, root_name, root_src, n_headers, names, contents, double_precision, address_width,
+                            nullptr, out);
+}
+
 extern "C" int ht_prog_precompile(const char *root_name, const char *root_src, int n_headers,
                                   const char *const *names, const char *const *contents, int double_precision,
-                                  int address_width) {
+                                  int address_width, const char *fp_options) {
     std::vector<char> code;
-    return get_code_object(root_name, root_src, n_headers, names, contents, double_precision, address_width, &code);
+    return get_code_object(root_name, root_src, n_headers, names, contents, double_precision, address_width,
+                           fp_options, &code);
 }
 
 extern "C" void ht_prog_destroy(ht_prog *prog) {

@@ -20,9 +20,12 @@ SIGNATURES = {
     "ht_ctx_sync": (c_int, [c_void_p]),
     "ht_prog_build": (c_int, [c_void_p, c_char_p, c_char_p, c_int, POINTER(c_char_p), POINTER(c_char_p), c_int,
                               c_int, POINTER(c_void_p)]),
+    "ht_prog_build_ex": (c_int, [c_void_p, c_char_p, c_char_p, c_int, POINTER(c_char_p), POINTER(c_char_p), c_int,
+                                 c_int, c_char_p, POINTER(c_void_p)]),
     "ht_prog_destroy": (None, [c_void_p]),
     "ht_prog_cache_hit": (c_int, []),
-    "ht_prog_precompile": (c_int, [c_char_p, c_char_p, c_int, POINTER(c_char_p), POINTER(c_char_p), c_int, c_int]),
+    "ht_prog_precompile": (c_int, [c_char_p, c_char_p, c_int, POINTER(c_char_p), POINTER(c_char_p), c_int, c_int,
+                                   c_char_p]),
     "ht_buf_alloc": (c_int, [c_void_p, c_size_t, POINTER(c_void_p)]),
     "ht_buf_free": (c_int, [c_void_p]),
     "ht_buf_size": (c_size_t, [c_void_p]),

@@ -181,17 +181,19 @@ def seed_fill(n: int, mode: int = 0, seed: int = REFERENCE_SEED) -> np.ndarray:
 class Program:
     """A compiled render program for one scene type."""
 
-    def __init__(self, ctx: Context, scene: SceneImpl, name: str = "render.hip"):
+    def __init__(self, ctx: Context, scene: SceneImpl, name: str = "render.hip", fp_options: Optional[str] = None):
         self.ctx = ctx
+        self.fp_options = fp_options
         self.source = codegen.root_source(scene, ctx.config)
         hdrs = codegen.device_headers()
         names = native.cstr_array(list(hdrs.keys()))
         contents = native.cstr_array(list(hdrs.values()))
         h = c_void_p()
         lib = native.lib()
-        native.check(lib.ht_prog_build(ctx.handle, name.encode(), self.source.encode(), len(hdrs), names, contents,
-                                       1 if ctx.config.double_support else 0, ctx.config.address_width,
-                                       ctypes.byref(h)))
+        native.check(lib.ht_prog_build_ex(ctx.handle, name.encode(), self.source.encode(), len(hdrs), names,
+                                          contents, 1 if ctx.config.double_support else 0,
+                                          ctx.config.address_width,
+                                          fp_options.encode() if fp_options is not None else None, ctypes.byref(h)))
         self.cache_hit = bool(lib.ht_prog_cache_hit())
         self.handle = h
 
@@ -201,7 +203,12 @@ class Program:
             self.handle = None
 
 
-def precompile(scene: SceneImpl, config: Config, name: str = "render.hip") -> bool:
+# Program FP modes (see include/hypertrace_hip.h, ht_prog_build_ex).
+STRICT_FP = "-ffp-contract=off -DHT_STRICT_FP=1"  # the default
+FAST_FP = "-fno-hip-fp32-correctly-rounded-divide-sqrt"
+
+
+def precompile(scene: SceneImpl, config: Config, name: str = "render.hip", fp_options: Optional[str] = None) -> bool:
     """Compile a scene's program into the code-object cache (no GPU needed).
     Returns True if it was already cached."""
     source = codegen.root_source(scene, config)
@@ -210,7 +217,8 @@ def precompile(scene: SceneImpl, config: Config, name: str = "render.hip") -> bo
     native.check(lib.ht_prog_precompile(name.encode(), source.encode(), len(hdrs),
                                         native.cstr_array(list(hdrs.keys())),
                                         native.cstr_array(list(hdrs.values())),
-                                        1 if config.double_support else 0, config.address_width))
+                                        1 if config.double_support else 0, config.address_width,
+                                        fp_options.encode() if fp_options is not None else None))
     return bool(lib.ht_prog_cache_hit())
 
 
@@ -218,9 +226,9 @@ class Render:
     """Render program of a scene type (render.rs:8-119). `render` runs one
     synchronous pass; `render_passes` queues several passes in one launch."""
 
-    def __init__(self, ctx: Context, scene: SceneImpl):
+    def __init__(self, ctx: Context, scene: SceneImpl, fp_options: Optional[str] = None):
         self.ctx = ctx
-        self.program = Program(ctx, scene)
+        self.program = Program(ctx, scene, fp_options=fp_options)
 
     def render(self, scene_buffer: EntityBuffer, canvas: Canvas) -> None:
         native.check(native.lib().ht_render(self.program.handle, canvas.width, canvas.height, scene_buffer.handle,

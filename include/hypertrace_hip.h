@@ -50,6 +50,16 @@ int ht_ctx_sync(ht_ctx *ctx);
 int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
                   const char *const *names, const char *const *contents, int double_precision,
                   int address_width, ht_prog **out);
+/* As ht_prog_build, with the floating-point code-generation options given
+ * explicitly (space-separated compiler flags). NULL = the default,
+ * "-ffp-contract=off -DHT_STRICT_FP=1": every operation rounded on its own,
+ * IEEE division and square root, FP32 transcendentals correctly rounded.
+ * "-fno-hip-fp32-correctly-rounded-divide-sqrt" is the fast mode: FMA
+ * contraction, reciprocal division, the FP32 device libm. No reference
+ * counterpart beyond ocl::Program build options. */
+int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
+                     const char *const *names, const char *const *contents, int double_precision,
+                     int address_width, const char *fp_options, ht_prog **out);
 void ht_prog_destroy(ht_prog *prog);
 /* 1 if the last build on this thread was served from the cache. */
 int ht_prog_cache_hit(void);
@@ -57,7 +67,7 @@ int ht_prog_cache_hit(void);
  * counterpart: lets a build step prepare programs ahead of time. */
 int ht_prog_precompile(const char *root_name, const char *root_src, int n_headers,
                        const char *const *names, const char *const *contents, int double_precision,
-                       int address_width);
+                       int address_width, const char *fp_options);
 
 /* Device buffers. Replace ocl::Buffer use in processing/src/buffer.rs:8-34
  * (EntityBuffer) and canvas.rs:125-189 (ImageBuffer). Writes copy the host

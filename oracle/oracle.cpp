@@ -14,7 +14,10 @@
 //                    colours FP32, kernel/ocl/algebra/color.hh:6-7)
 //   <Cnt, Cnt>       flop counting: add/sub/mul/div/sqrt = 1, each
 //                    transcendental = 1, comparisons/negation/abs/min/max = 0
-// Built with -ffp-contract=off so every operation rounds on its own.
+// Built with -ffp-contract=off so every operation rounds on its own; FP32
+// transcendentals are evaluated in FP64 and rounded once (correctly rounded
+// except in ~2^-28 of cases), so the oracle does not carry one libm's ulp
+// errors. The reference leaves these to its OpenCL implementation's builtins.
 //
 // Parity status per part (which check pins it) is listed in the header of
 // oracle/README.md.
@@ -65,16 +68,16 @@ inline bool operator==(Cnt a, Cnt b) { return a.v == b.v; }
 inline float fsqrt(float x) { return std::sqrt(x); }
 inline double fsqrt(double x) { return std::sqrt(x); }
 inline Cnt fsqrt(Cnt x) { ++g_flops; return Cnt(std::sqrt(x.v)); }
-inline float flog(float x) { return std::log(x); }
+inline float flog(float x) { return (float)std::log((double)x); }
 inline double flog(double x) { return std::log(x); }
 inline Cnt flog(Cnt x) { ++g_flops; return Cnt(std::log(x.v)); }
-inline float fcos(float x) { return std::cos(x); }
+inline float fcos(float x) { return (float)std::cos((double)x); }
 inline double fcos(double x) { return std::cos(x); }
 inline Cnt fcos(Cnt x) { ++g_flops; return Cnt(std::cos(x.v)); }
-inline float fsin(float x) { return std::sin(x); }
+inline float fsin(float x) { return (float)std::sin((double)x); }
 inline double fsin(double x) { return std::sin(x); }
 inline Cnt fsin(Cnt x) { ++g_flops; return Cnt(std::sin(x.v)); }
-inline float fpow(float a, float b) { return std::pow(a, b); }
+inline float fpow(float a, float b) { return (float)std::pow((double)a, (double)b); }
 inline double fpow(double a, double b) { return std::pow(a, b); }
 inline Cnt fpow(Cnt a, Cnt b) { ++g_flops; return Cnt(std::pow(a.v, b.v)); }
 inline float fabs_(float x) { return std::fabs(x); }
