@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpaths/s + time-to-RMSE 0.01 vs FP64 oracle: chosen form vs plain kernel"
 FP32_VECTOR_PEAK = 157.3  # TFLOP/s, MI355X published FP32 vector (packed / dual-issue)
 FP32_NONPACKED_PEAK = 78.6  # TFLOP/s without packed FP32 (this project compiles packed FP32 off)
+# HBM bytes per render_passes launch on this workload, from the FETCH_SIZE /
+# WRITE_SIZE counter runs scaled by the calibration copy
+# (profiles/render_passes_hyperbolic_cells_strict.md): 6 615 KiB + 6 000 KiB.
+MEASURED_TRAFFIC_BYTES = (6615 + 6000) * 1024
 
 
 def count_flops_per_path(scene_spec, w, h, seeds, stride=16):
@@ -69,7 +73,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="time only the chosen form")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "
+                         "the known-size copy that scales FETCH_SIZE / WRITE_SIZE counter readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the render pass shards by "
@@ -118,6 +125,13 @@ def main():
                     "value": paths_per_step * args.steps / fwall / 1e6, "unit": "Mpaths/s",
                     "kernel_ms": fgpu / args.steps}
 
+        if args.calibrate:
+            from hypertrace_amd.processing import Extractor, ImageBuffer
+            big = Canvas(ctx, (4096, 4096))
+            big.passes = 1
+            Extractor(ctx).process(big, ImageBuffer(ctx, (4096, 4096)))
+            ctx.sync()
+
         plain = None
         if not args.no_plain:
             canvas_p = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
@@ -140,7 +154,8 @@ def main():
                 "frac": achieved / FP32_VECTOR_PEAK, "frac_nonpacked": achieved / FP32_NONPACKED_PEAK,
                 "kernel": "render_passes", "kernel_ms": kernel_ms,
                 "flops_per_path": flops_per_path, "flops_sample_paths": sample_paths,
-                "traffic": None}
+                "traffic": MEASURED_TRAFFIC_BYTES,
+                "algorithmic_bytes": 40 * W * H + len(wl.scene.to_bytes(X32_F32))}
 
     out = {
         "metric": METRIC,
