@@ -65,9 +65,21 @@ def layout_asserts(scene: SceneImpl, cfg: Config) -> List[str]:
     return lines
 
 
+def bounds_cap(scene: SceneImpl) -> int:
+    """LDS capacity (elements) of the culled scan: the next power of two
+    holding the scene's top-level vector, at least 64. A later scene of the
+    same type with more elements falls back to the plain scan."""
+    n = len(scene.object.items) if isinstance(scene.object, _VecBase) else 0
+    cap = 64
+    while cap < n:
+        cap *= 2
+    return min(cap, 4096)
+
+
 def root_source(scene: SceneImpl, cfg: Config) -> str:
     lines = [
         "// Generated render program for one scene type.",
+        f"#define HT_BOUNDS_CAP {bounds_cap(scene)}",
         "#include <ht/render.hh>",
         f"typedef {scene.cpp()} Scene;",
     ]

@@ -470,6 +470,7 @@ template <class G> struct CoveredCache { typename G::Dir normal; };
 
 template <class G, class S, class Mat> struct Covered {
     typedef Fields<Mat, S> L;
+    typedef S ShapeT;
     static constexpr size_t_ SIZE = L::size(), ALIGN = L::align();
     static constexpr bool DYN = false;
     typedef CoveredCache<G> Cache;
@@ -689,29 +690,6 @@ template <class G, class V, class T, class B, int H> struct SceneImpl {
 
     static HD void bg(const u8 *self, Context *ctx, const Light<G> *light, color3 *emission) {
         B::bg_interact(background(self), ctx, light, emission);
-    }
-
-    static HD color3 sample(const u8 *self, Context *ctx, real2 pix_pos, real2 pix_size) {
-        Light<G> light;
-        light.base.intensity = c3(1.0f);
-        light.base.diffuse = false;
-        light.ray = V::sample(view(self), ctx, pix_pos, pix_size);
-        color3 emission = c3(0.0f);
-        ctx->prev_hash = hash_never();
-        for (int i = 0; i < H; ++i) {
-            ctx->hasher = hash_init();
-            typename T::Cache cache;
-            real dist = T::detect(object(self), ctx, &cache, &light);
-            if (dist > (real)-0.5f) {
-                bool ret = T::interact(object(self), ctx, &cache, &light, &emission);
-                if (!ret) break;
-                ctx->prev_hash = hash_finish(&ctx->hasher);
-            } else {
-                bg(self, ctx, &light, &emission);
-                break;
-            }
-        }
-        return emission;
     }
 };
 

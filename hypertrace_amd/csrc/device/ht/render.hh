@@ -7,12 +7,15 @@
 // render_passes  `passes` consecutive passes in one launch. Each thread owns a
 //              pixel and, when its path ends, starts the pixel's next sample
 //              in place (path regeneration), so the wavefront keeps running the
-//              wavefront-uniform nearest-hit scan on full lanes. Per pixel the
-//              sample sequence, the RNG stream and the order of the canvas
-//              additions are the same as `passes` calls of `render`, so the
-//              canvas and seeds come out identical.
+//              nearest-hit scan on full lanes. Per pixel the sample sequence,
+//              the RNG stream and the order of the canvas additions are the
+//              same as `passes` calls of `render`, so the canvas and seeds come
+//              out identical.
+// Both run the bounce loop of render/scene.inl:16-44; the top-level object's
+// nearest-hit scan goes through TopScan (scan.hh), which culls and compacts
+// the exact tests where the scene allows it without changing any result.
 #pragma once
-#include "entities.hh"
+#include "scan.hh"
 
 namespace ht {
 
@@ -30,9 +33,40 @@ HD PixelSetup pixel_setup(uint x, uint y, uint w, uint h) {
     return p;
 }
 
+// One bounce of scene.inl:25-41. Returns true when the path ends here.
+template <class S>
+HD bool bounce(const u8 *scene, Context *ctx, Light<typename S::Geo> *light, color3 *emission) {
+    typedef typename S::Object T;
+    const u8 *obj = S::object(scene);
+    ctx->hasher = hash_init();
+    typename T::Cache cache;
+    real dist = TopScan<T>::detect(obj, ctx, &cache, light);
+    if (dist > (real)-0.5f) {
+        bool ret = T::interact(obj, ctx, &cache, light, emission);
+        if (!ret) return true;
+        ctx->prev_hash = hash_finish(&ctx->hasher);
+        return false;
+    }
+    S::bg(scene, ctx, light, emission);
+    return true;
+}
+
+template <class S>
+HD void start_path(const u8 *scene, Context *ctx, const PixelSetup &px, Light<typename S::Geo> *light,
+                   color3 *emission) {
+    light->base.intensity = c3(1.0f);
+    light->base.diffuse = false;
+    light->ray = S::View::sample(S::view(scene), ctx, px.pos, px.size);
+    *emission = c3(0.0f);
+    ctx->prev_hash = hash_never();
+}
+
 template <class S>
 HD void render_pass_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
                          uint *__restrict__ seeds) {
+    typedef typename S::Geo G;
+    TopScan<typename S::Object>::prepare(S::object(scene));
+    __syncthreads();
     uint idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= w * h) return;
     uint x = idx % w, y = idx / w;
@@ -40,12 +74,16 @@ HD void render_pass_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *_
     Context ctx;
     ctx.rng.state = seeds[idx];
     ctx.hasher = hash_init();
-    ctx.prev_hash = hash_never();
-    color3 c = S::sample(scene, &ctx, px.pos, px.size);
+    Light<G> light;
+    color3 emission;
+    start_path<S>(scene, &ctx, px, &light, &emission);
+    for (int i = 0; i < S::HOPS; ++i) {
+        if (bounce<S>(scene, &ctx, &light, &emission)) break;
+    }
     float4 acc = canvas[idx];
-    acc.x += c.x;
-    acc.y += c.y;
-    acc.z += c.z;
+    acc.x += emission.x;
+    acc.y += emission.y;
+    acc.z += emission.z;
     acc.w += 1.0f;
     canvas[idx] = acc;
     seeds[idx] = ctx.rng.state;
@@ -55,8 +93,9 @@ template <class S>
 HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
                            uint *__restrict__ seeds, uint passes) {
     typedef typename S::Geo G;
-    typedef typename S::Object T;
     constexpr int H = S::HOPS;
+    TopScan<typename S::Object>::prepare(S::object(scene));
+    __syncthreads();
     uint idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= w * h) return;
     uint x = idx % w, y = idx / w;
@@ -64,44 +103,27 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
     Context ctx;
     ctx.rng.state = seeds[idx];
     ctx.hasher = hash_init();
-    ctx.prev_hash = hash_never();
     float4 acc = canvas[idx];
-    const u8 *obj = S::object(scene);
 
+    Light<G> light;
+    color3 emission = c3(0.0f);
     if constexpr (H <= 0) {
         for (uint k = 0; k < passes; ++k) {
-            (void)S::sample(scene, &ctx, px.pos, px.size);
+            start_path<S>(scene, &ctx, px, &light, &emission);
             acc.w += 1.0f;
         }
     } else {
-        Light<G> light;
-        color3 emission = c3(0.0f);
         int hop = 0;
         uint done = 0;
         bool start = true;
         for (;;) {
             if (start) {
                 if (done == passes) break;
-                light.base.intensity = c3(1.0f);
-                light.base.diffuse = false;
-                light.ray = S::View::sample(S::view(scene), &ctx, px.pos, px.size);
-                emission = c3(0.0f);
-                ctx.prev_hash = hash_never();
+                start_path<S>(scene, &ctx, px, &light, &emission);
                 hop = 0;
                 start = false;
             }
-            ctx.hasher = hash_init();
-            typename T::Cache cache;
-            real dist = T::detect(obj, &ctx, &cache, &light);
-            bool end;
-            if (dist > (real)-0.5f) {
-                bool ret = T::interact(obj, &ctx, &cache, &light, &emission);
-                if (ret) ctx.prev_hash = hash_finish(&ctx.hasher);
-                end = !ret;
-            } else {
-                S::bg(scene, &ctx, &light, &emission);
-                end = true;
-            }
+            bool end = bounce<S>(scene, &ctx, &light, &emission);
             ++hop;
             if (hop >= H) end = true;
             if (end) {

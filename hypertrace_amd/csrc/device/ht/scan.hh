@@ -1,0 +1,432 @@
+// Culled, wavefront-compacted nearest-hit scan over a vector of mapped
+// objects, with results identical to the plain scan (object/vector.inl:24-62).
+//
+// 1. Bounds. At kernel start every block computes, for each element, a
+//    Euclidean sphere that contains the world image of its primitive and
+//    stages it in LDS (16 B per element): a unit sphere / ball / cube mapped
+//    by an isometry, or, in the upper half-space, the image of the unit
+//    hemisphere (a hemisphere over the image of the unit circle) or of the
+//    horosphere z = 1 (a horoball) under the element's Moebius map. Images
+//    that pass through infinity (a plane) get no bound.
+// 2. Cull. Per ray and element, an FP64 test against the geodesic: the
+//    straight ray in Euclidean space; in the half-space the semicircle in the
+//    vertical plane of the ray, restricted to the part ahead of the start.
+//    The sphere is inflated by a margin far above the rounding error of the
+//    element's own exact test, so a culled element is one whose exact test
+//    returns a miss; the exact test is skipped only where its result cannot
+//    matter.
+// 3. Compaction. The surviving (ray, element) pairs of a wavefront are
+//    appended to an LDS queue with a ballot and a lane prefix count, and
+//    processed 64 at a time on full lanes: each lane runs the exact test of
+//    one pair for another lane's ray and folds the distance into that lane's
+//    slot with a 64-bit LDS atomic min on (ordered distance, element index),
+//    which selects the first element with the smallest distance, as the
+//    reference loop does. Each lane then reruns its winner's exact test to
+//    get the hit state, and maps it back to the parent frame once.
+#pragma once
+#include "entities.hh"
+
+#ifndef HT_BOUNDS_CAP
+#define HT_BOUNDS_CAP 256
+#endif
+
+namespace ht {
+
+static constexpr int BOUNDS_CAP = HT_BOUNDS_CAP;
+static constexpr int WAVE = 64;
+static constexpr int MAX_WAVES = 4;  // blocks of 256 threads
+
+// World-space sphere of an element: r >= the outer radius that contains the
+// primitive's image with margin (r < 0: no bound, always a candidate);
+// ri <= an inner radius that the image contains with margin (used only for
+// "the geodesic stays inside the sphere", where shrinking is the
+// conservative direction).
+struct Bound {
+    float cx, cy, cz, r;
+    float ri, horo, pad1, pad2;  // horo = 1: a horoball, tangent to the boundary below its centre
+};
+
+// Relative cull margin for an element whose exact test runs through a map
+// with squared Frobenius norm n2: generous against the FP32 rounding of the
+// mapped ray (relative error grows with the map's conditioning).
+HD double cull_margin(double n2) { return 1e-3 + 1e-4 * n2 * n2; }
+
+// ------------------------------------------------------------ local kinds
+// Primitive kind inside a Covered object: 0 = none (no bound), 1 = unit
+// sphere, 2 = ball (radius in the data), 3 = unit cube, 4 = unit hemisphere
+// (hyperbolic plane), 5 = horosphere z = 1.
+template <class S> struct PrimKind { static HD int kind(const u8 *) { return 0; } static HD double radius(const u8 *) { return 0.0; } };
+template <> struct PrimKind<Sphere> { static HD int kind(const u8 *) { return 1; } static HD double radius(const u8 *) { return 1.0; } };
+template <> struct PrimKind<Ball> { static HD int kind(const u8 *) { return 2; } static HD double radius(const u8 *s) { return (double)ld<real>(s); } };
+template <> struct PrimKind<Cube> { static HD int kind(const u8 *) { return 3; } static HD double radius(const u8 *) { return 1.7320508075688772; } };
+template <> struct PrimKind<Plane<Hy>> { static HD int kind(const u8 *) { return 4; } static HD double radius(const u8 *) { return 1.0; } };
+template <> struct PrimKind<Horosphere> { static HD int kind(const u8 *) { return 5; } static HD double radius(const u8 *) { return 1.0; } };
+
+template <class G, class... Vs> struct PrimKind<ShapeChoice<G, Vs...>> {
+    typedef ChoiceLayout<Vs...> C;
+    template <usize I, class V, class... Rest> static HD int kind_at(usize idx, const u8 *d) {
+        if (idx == I) return PrimKind<V>::kind(d);
+        if constexpr (sizeof...(Rest) > 0) return kind_at<I + 1, Rest...>(idx, d); else return 0;
+    }
+    template <usize I, class V, class... Rest> static HD double radius_at(usize idx, const u8 *d) {
+        if (idx == I) return PrimKind<V>::radius(d);
+        if constexpr (sizeof...(Rest) > 0) return radius_at<I + 1, Rest...>(idx, d); else return 0.0;
+    }
+    static HD int kind(const u8 *s) { return kind_at<0, Vs...>(C::index(s), C::data(s)); }
+    static HD double radius(const u8 *s) { return radius_at<0, Vs...>(C::index(s), C::data(s)); }
+};
+
+// Elements the culled scan handles: Mapped<G, Covered<G, S, Mat>, M>.
+template <class T> struct Cullable { static constexpr bool value = false; };
+template <class G, class S, class Mat, class M> struct Cullable<Mapped<G, Covered<G, S, Mat>, M>> {
+    static constexpr bool value = true;
+    typedef Covered<G, S, Mat> CoveredT;
+    static HD const u8 *shape(const u8 *elem) {
+        return Mapped<G, CoveredT, M>::inner(elem) + CoveredT::L::offset(1);
+    }
+};
+
+// ------------------------------------------------------------ world bounds
+HD Bound no_bound() { return {0.f, 0.f, 0.f, -1.f, -1.f, 0.f, 0.f, 0.f}; }
+
+// Outer radius inflated, inner radius shrunk, in FP64, each rounded
+// outwards (resp. inwards) to FP32 together with the centre's rounding.
+HD Bound make_bound(double cx, double cy, double cz, double r, double margin) {
+    float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+    double err = fabs_((double)fx - cx) + fabs_((double)fy - cy) + fabs_((double)fz - cz);
+    double ro = r * (1.0 + margin) + err + 1e-30;
+    float fo = (float)ro;
+    if ((double)fo < ro) fo = __uint_as_float(__float_as_uint(fo) + 1u);  // next float up (fo > 0)
+    double rin = r * (1.0 - margin) - err;
+    float fi = rin > 0.0 ? (float)rin : 0.0f;
+    if ((double)fi > rin && fi > 0.0f) fi = __uint_as_float(__float_as_uint(fi) - 1u);  // next float down
+    return {fx, fy, fz, fo, fi, 0.f, 0.f, 0.f};
+}
+
+template <class M> struct WorldBound;
+
+template <> struct WorldBound<Moebius> {
+    static HD Bound of(const Moebius &m, int kind, double) {
+        double ar = m.v.a.x, ai = m.v.a.y, br = m.v.b.x, bi = m.v.b.y;
+        double cr = m.v.c.x, ci = m.v.c.y, dr = m.v.d.x, di = m.v.d.y;
+        double n2 = ar * ar + ai * ai + br * br + bi * bi + cr * cr + ci * ci + dr * dr + di * di;
+        double margin = cull_margin(n2);
+        if (kind == 5) {
+            // horosphere z = 1 -> horoball tangent at a/c, diameter 1/|c|^2
+            double c2 = cr * cr + ci * ci;
+            if (!(c2 > 1e-12 * n2)) return no_bound();
+            double px = (ar * cr + ai * ci) / c2, py = (ai * cr - ar * ci) / c2;
+            double rad = 0.5 / c2;
+            Bound b = make_bound(px, py, rad, rad, margin);
+            b.horo = 1.f;
+            return b;
+        }
+        if (kind == 4) {
+            // unit hemisphere -> hemisphere over f(unit circle), f(w) = (aw+b)/(cw+d):
+            // centre (b conj(d) - a conj(c)) / (|d|^2 - |c|^2), radius 1 / ||d|^2 - |c|^2|
+            double den = (dr * dr + di * di) - (cr * cr + ci * ci);
+            if (!(fabs_(den) > 1e-9 * n2)) return no_bound();
+            double nx = (br * dr + bi * di) - (ar * cr + ai * ci);
+            double ny = (bi * dr - br * di) - (ai * cr - ar * ci);
+            return make_bound(nx / den, ny / den, 0.0, 1.0 / fabs_(den), margin);
+        }
+        return no_bound();
+    }
+};
+
+template <> struct WorldBound<Homogenous3> {
+    static HD Bound of(const Homogenous3 &m, int kind, double radius) {
+        if (kind < 1 || kind > 3) return no_bound();
+        double cx = m.outer.v.x, cy = m.outer.v.y, cz = m.outer.v.z;  // rotation fixes the origin
+        double dist = fabs_(cx) + fabs_(cy) + fabs_(cz);
+        return make_bound(cx, cy, cz, radius + 1e-4 * dist, 1e-4);
+    }
+};
+
+template <> struct WorldBound<Shift3> {
+    static HD Bound of(const Shift3 &m, int kind, double radius) {
+        if (kind < 1 || kind > 3) return no_bound();
+        double cx = m.v.x, cy = m.v.y, cz = m.v.z;
+        double dist = fabs_(cx) + fabs_(cy) + fabs_(cz);
+        return make_bound(cx, cy, cz, radius + 1e-4 * dist, 1e-4);
+    }
+};
+
+template <class T, class M> HD Bound element_bound(const u8 *elem) {
+    typedef Cullable<T> CT;
+    typedef typename CT::CoveredT Cov;
+    const u8 *shape = CT::shape(elem);
+    int kind = PrimKind<typename Cov::ShapeT>::kind(shape);
+    M m = T::map(elem);
+    return WorldBound<M>::of(m, kind, PrimKind<typename Cov::ShapeT>::radius(shape));
+}
+
+// ------------------------------------------------------------ ray cull tests
+// Euclidean ray p + t d, t >= 0.
+struct CullRayEu {
+    double px, py, pz, dx, dy, dz;
+    bool valid;
+    static HD CullRayEu make(const Ray<Eu> &r) {
+        return {(double)r.start.x, (double)r.start.y, (double)r.start.z,
+                (double)r.direction.x, (double)r.direction.y, (double)r.direction.z, true};
+    }
+    HD bool candidate(const Bound &b) const {
+        if (b.r < 0.f || !valid) return true;
+        double sx = b.cx - px, sy = b.cy - py, sz = b.cz - pz;
+        double r = b.r;
+        double t = sx * dx + sy * dy + sz * dz;
+        if (t < -r) return false;  // wholly behind the start
+        double qx = sx - t * dx, qy = sy - t * dy, qz = sz - t * dz;
+        return qx * qx + qy * qy + qz * qz <= r * r;
+    }
+};
+
+// Hyperbolic geodesic from p with unit tangent d: the circle centred on the
+// boundary plane in the vertical plane through p along u = d.xy / |d.xy|,
+// centre at p.xy + gam u (gam = p.z d.z / |d.xy|), radius p.z / |d.xy|; the
+// part ahead of p is the part with horizontal coordinate >= 0 along u.
+struct CullRayHy {
+    double px, py, pz, dx, dy, dz, ux, uy, gam, rg;
+    bool valid;
+    static HD CullRayHy make(const Ray<Hy> &r) {
+        CullRayHy c;
+        double dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
+        double pz = r.start.z;
+        double dxy = sqrt_(dx * dx + dy * dy);
+        double dn = sqrt_(dx * dx + dy * dy + dz * dz);
+        c.valid = dxy > 1e-4 * dn && pz > 0.0;
+        if (!c.valid) dxy = 1.0;
+        c.px = r.start.x;
+        c.py = r.start.y;
+        c.pz = pz;
+        c.dx = dx / dn;
+        c.dy = dy / dn;
+        c.dz = dz / dn;
+        c.ux = dx / dxy;
+        c.uy = dy / dxy;
+        c.gam = pz * dz / dxy;
+        c.rg = pz * dn / dxy;
+        return c;
+    }
+    HD bool toward_tangent_point(double tx, double ty) const;
+    HD bool candidate(const Bound &b) const {
+        if (b.r < 0.f || !valid) return true;
+        double rx = (double)b.cx - px, ry = (double)b.cy - py;
+        double ro = b.r, ri = b.ri;
+        double delta = -rx * uy + ry * ux;  // distance from the vertical plane
+        double d2p = delta * delta;
+        double rp2 = ro * ro - d2p;  // squared radius of the plane's cut of the outer sphere
+        if (rp2 < 0.0) return false;
+        double sig = rx * ux + ry * uy;  // along the plane, from p
+        double sz = b.cz;
+        // the cut circle has a point ahead of p (horizontal coordinate >= 0);
+        // not applied to a horoball when the ray is near the geodesic through
+        // its tangent point T: there the element's local ray is near vertical,
+        // and the horosphere test (shape/hy/horosphere.cc:16-24) applies its
+        // EPS window before dividing by |d.xy|^2, so it accepts crossings
+        // behind the start.
+        if (sig < 0.0 && sig * sig > rp2 && !(b.horo > 0.f && toward_tangent_point(b.cx, b.cy))) return false;
+        double ex = sig - gam;
+        double d2 = ex * ex + sz * sz;  // squared distance between the circle centres
+        double rg2 = rg * rg;
+        // disjoint: d2 > (rg + rp)^2  <=>  d2 - rg^2 - rp^2 > 2 rg rp
+        double o = d2 - rg2 - rp2;
+        if (o > 0.0 && o * o > 4.0 * rg2 * rp2) return false;
+        // outer cut circle strictly inside the geodesic circle: rp < rg, d2 < (rg - rp)^2
+        if (rg2 > rp2) {
+            double in = rg2 + rp2 - d2;
+            if (in > 0.0 && in * in > 4.0 * rg2 * rp2) return false;
+        }
+        // geodesic circle strictly inside the inner sphere's cut: rg < rpi, d2 < (rpi - rg)^2
+        double rpi2 = ri * ri - d2p;
+        if (rpi2 > rg2) {
+            double in = rg2 + rpi2 - d2;
+            if (in > 0.0 && in * in > 4.0 * rg2 * rpi2) return false;
+        }
+        return true;
+    }
+};
+
+// True if the ray direction is within ~6 degrees of the tangent (either
+// sense) of the geodesic from p to the boundary point T = (tx, ty, 0): the
+// angle to the local vertical of a horoball tangent at T (maps are conformal).
+HD bool CullRayHy::toward_tangent_point(double tx, double ty) const {
+    double vx = tx - px, vy = ty - py;
+    double v2 = vx * vx + vy * vy, h = pz;
+    // tangent at p of the circle through p and T orthogonal to the boundary,
+    // scaled by |v|: (h v, (|v|^2 - h^2) / 2)
+    double tz = 0.5 * (v2 - h * h);
+    double n = h * (dx * vx + dy * vy) + dz * tz;
+    double den2 = h * h * v2 + tz * tz;
+    return den2 - n * n < 1e-2 * den2;
+}
+
+template <class G> struct CullRayOf;
+template <> struct CullRayOf<Eu> { typedef CullRayEu type; };
+template <> struct CullRayOf<Hy> { typedef CullRayHy type; };
+
+// ------------------------------------------------------------ LDS staging
+template <class G> struct WaveScratch {
+    Ray<G> ray[WAVE];
+    uint orig_hasher[WAVE];
+    uint prev_hash[WAVE];
+    unsigned long long best[WAVE];
+    uint queue[2 * WAVE];
+};
+
+template <class Tag> HD Bound *bounds_lds() {
+    __shared__ Bound b[BOUNDS_CAP];
+    return b;
+}
+template <class G> HD WaveScratch<G> *wave_scratch_lds() {
+    __shared__ WaveScratch<G> s[MAX_WAVES];
+    return s;
+}
+
+HD void wave_sync() {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+HD unsigned long long lane_mask_below() {
+    uint lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Monotonic unsigned key of a float.
+HD uint order_key(float f) {
+    uint u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// ------------------------------------------------------------ the scan
+// Top-level object of a scene: the culled scan for a vector of cullable
+// elements in FP32 programs, the element-by-element scan otherwise.
+template <class T> struct TopScan {
+    static HD void prepare(const u8 *) {}
+    template <class G>
+    static HD real detect(const u8 *self, Context *ctx, typename T::Cache *cache, Light<G> *light) {
+        return T::detect(self, ctx, cache, light);
+    }
+};
+
+template <class G, class E> struct TopScan<ObjectVec<G, E>> {
+    typedef ObjectVec<G, E> OV;
+    typedef typename OV::V V;
+#if defined(HT_DOUBLE) || defined(HT_NO_CULL)
+    // FP64: the packed (distance, index) key is FP32. HT_NO_CULL: the plain
+    // scan on request (for comparisons).
+    static constexpr bool CULL = false;
+#else
+    static constexpr bool CULL = Cullable<E>::value;
+#endif
+
+    // All threads of the block, before any of them diverges; the caller
+    // synchronises the block afterwards.
+    static HD void prepare(const u8 *self) {
+        if constexpr (CULL) {
+            usize n = V::len(self);
+            if (n > (usize)BOUNDS_CAP) return;
+            Bound *b = bounds_lds<OV>();
+            for (uint i = threadIdx.x; i < (uint)n; i += blockDim.x) {
+                b[i] = element_bound<E, typename E::Map>(V::elem(self, i));
+            }
+        }
+    }
+
+    // Exact test of element i for a ray, with the hasher state the reference
+    // loop would have at that element.
+    static HD real element_test(const u8 *self, usize i, Ray<G> ray, uint orig_hasher, uint prev_hash,
+                                typename E::Cache *cc, Ray<G> *local_out) {
+        const u8 *elem = V::elem(self, i);
+        Context c2;
+        c2.rng.state = 0;
+        c2.hasher.state = orig_hasher;
+        hash_usize(&c2.hasher, i);
+        c2.prev_hash = prev_hash;
+        Light<G> l;
+        l.ray = ray_map<G>(ray, E::map(elem).inverse());
+        real dist = E::Inner::detect(E::inner(elem), &c2, cc, &l);
+        *local_out = l.ray;
+        return dist;
+    }
+
+    static HD void process(const u8 *self, WaveScratch<G> *ws, uint arank, uint k) {
+        if (arank < k) {
+            uint e = ws->queue[arank];
+            uint i = e >> 6, L = e & 63u;
+            typename E::Cache cc;
+            Ray<G> local;
+            real dist = element_test(self, i, ws->ray[L], ws->orig_hasher[L], ws->prev_hash[L], &cc, &local);
+            if (dist > -EPS) {
+                unsigned long long key = ((unsigned long long)order_key((float)dist) << 32) | (unsigned long long)i;
+                atomicMin(&ws->best[L], key);
+            }
+        }
+    }
+
+    static HD real detect(const u8 *self, Context *ctx, typename OV::Cache *cache, Light<G> *light) {
+        if constexpr (!CULL) {
+            return OV::detect(self, ctx, cache, light);
+        } else {
+            usize n = V::len(self);
+            if (n > (usize)BOUNDS_CAP) return OV::detect(self, ctx, cache, light);
+            const Bound *bounds = bounds_lds<OV>();
+            WaveScratch<G> *ws = wave_scratch_lds<G>() + (threadIdx.x / WAVE);
+            uint lane = __lane_id();
+            uint orig = ctx->hasher.state;
+            ws->ray[lane] = light->ray;
+            ws->orig_hasher[lane] = orig;
+            ws->prev_hash[lane] = ctx->prev_hash;
+            ws->best[lane] = ~0ull;
+            unsigned long long below = lane_mask_below();
+            unsigned long long active = __ballot(1);
+            uint nact = (uint)__popcll(active);
+            uint arank = (uint)__popcll(active & below);
+            typename CullRayOf<G>::type cr = CullRayOf<G>::type::make(light->ray);
+            wave_sync();
+            uint qcount = 0;
+            for (usize i = 0; i < n; ++i) {
+                bool cand = cr.candidate(bounds[i]);
+                unsigned long long m = __ballot(cand);
+                if (m == 0) continue;
+                if (cand) ws->queue[qcount + (uint)__popcll(m & below)] = ((uint)i << 6) | lane;
+                qcount += (uint)__popcll(m);
+                if (qcount >= nact) {
+                    wave_sync();
+                    process(self, ws, arank, nact);
+                    uint rem = qcount - nact;
+                    uint v = 0;
+                    if (arank < rem) v = ws->queue[nact + arank];
+                    wave_sync();
+                    if (arank < rem) ws->queue[arank] = v;
+                    qcount = rem;
+                    wave_sync();
+                }
+            }
+            if (qcount > 0) {
+                wave_sync();
+                process(self, ws, arank, qcount);
+            }
+            wave_sync();
+            unsigned long long best = ws->best[lane];
+            if (best == ~0ull) {
+                ctx->hasher.state = orig;
+                return -R1;
+            }
+            usize i = (usize)(best & 0xffffffffull);
+            typename E::Cache cc;
+            Ray<G> local;
+            real dist = element_test(self, i, light->ray, orig, ctx->prev_hash, &cc, &local);
+            ctx->hasher.state = orig;
+            hash_usize(&ctx->hasher, i);
+            cache->inner = cc;
+            cache->index = i;
+            light->ray = ray_map<G>(local, E::map(V::elem(self, i)));
+            return dist;
+        }
+    }
+};
+
+}  // namespace ht

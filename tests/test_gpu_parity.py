@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from hypertrace_amd import scenes
-from hypertrace_amd.processing import (Canvas, Context, EntityBuffer, Extractor, GammaFilter, ImageBuffer, Packer,
+from hypertrace_amd.processing import (STRICT_FP, Canvas, Context, EntityBuffer, Extractor, GammaFilter, ImageBuffer, Packer,
                                        Pipeline, Render, seed_fill)
 from hypertrace_amd.types import X32_F32, X32_F64
 
@@ -155,3 +155,25 @@ def test_pipeline_fused_post_equals_separate(ctx):
     b = p.get_image(fused=True).copy()
     np.testing.assert_array_equal(a, b)
     assert np.all(a[..., 3] == 255)
+
+
+NO_CULL = STRICT_FP + " -DHT_NO_CULL=1"
+
+
+@pytest.mark.parametrize("name,full", [("eu_ball", False), ("eu_unit", False), ("hy_cells", False),
+                                       ("hy_cells", True)])
+def test_culled_scan_identical_to_plain_scan(ctx, name, full):
+    """The culled, compacted nearest-hit scan must give bit-identical canvas
+    and seeds to the element-by-element scan."""
+    wl = scenes.hyperbolic_cells() if full else WORKLOADS[name]()
+    print(f"case {name} full={full}", flush=True)
+    buf = EntityBuffer(ctx, wl.scene)
+    out = []
+    for fp in (None, NO_CULL):
+        r = Render(ctx, wl.scene, fp_options=fp)
+        c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
+        r.render_passes(buf, c, 4)
+        r.render(buf, c)
+        out.append((c.load_image(), c.load_seeds()))
+    np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
