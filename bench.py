@@ -2,17 +2,28 @@
 
 Workload: the hyperbolic cell scene, 640x480, 64 samples per pixel, 8 light
 hops, 192 primitives (64 cells x (1 horosphere + 2 geodesic planes)). A step
-is one full 64-spp render of the frame (19.66 M paths) with the pass-batching
-kernel (`render_passes`, one launch per step). Inputs (scene, canvas, seeds)
-are resident in HBM before the timed region. The plain per-pass kernel
-(`render`, one launch + sync per pass, as the reference runs) is timed beside
-it on the same frame.
+is one full 64-spp render of the frame (19.66 M paths) with the chosen form
+(`render_passes`: one launch per frame, in-thread path regeneration, culled
+and compacted nearest-hit scan, strict IEEE FP32 — bit-identical to the plain
+kernel). Inputs (scene, canvas, seeds) are resident in HBM before the timed
+region. Beside it:
 
-Prints one JSON line. `roofline` prices the dominant kernel's algorithmic
-FP32 flops (counted by the CPU oracle's counting scalar on a sample of the
-same paths) against the MI355X FP32 vector peak. `cpu_baseline` times the CPU
-oracle (a restatement of the reference kernel) on a crop of the same frame on
-the host's cores.
+* plain: the per-pass kernel (`render`, one launch + sync per pass, as the
+  reference runs it) with the element-by-element scan, same strict FP32;
+* time-to-RMSE: wall time until the image reaches RMSE <= 0.01 against an
+  FP64 4096-spp oracle image rendered on the GPU from independent seeds (RMSE
+  over all pixels x RGB of the extracted, gamma-1/2.2, clamped [0, 1] image),
+  chosen form vs plain kernel. Both forms produce bit-identical canvases for
+  the same number of passes, so the spp needed is found once and each form is
+  then timed over exactly that many passes;
+* roofline: the dominant kernel's algorithmic FP32 flops (counted by the CPU
+  oracle's counting scalar on a sample of the same paths: the work of the
+  reference algorithm, which the culled scan skips most of) per second of its
+  HIP-event time, against the MI355X FP32 vector peak;
+* cpu_baseline: the CPU oracle (a restatement of the reference kernel) on
+  rows of the same frame, on the host's cores.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -32,39 +43,125 @@ FP32_VECTOR_PEAK = 157.3  # TFLOP/s, MI355X published FP32 vector (packed / dual
 FP32_NONPACKED_PEAK = 78.6  # TFLOP/s without packed FP32 (this project compiles packed FP32 off)
 # HBM bytes per render_passes launch on this workload, from the FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by the calibration copy
-# (profiles/render_passes_hyperbolic_cells_strict.md): 6 615 KiB + 6 000 KiB.
+# (profiles/render_passes_hyperbolic_cells_strict.md).
 MEASURED_TRAFFIC_BYTES = (6615 + 6000) * 1024
+NO_CULL = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_NO_CULL=1"
+ORACLE_SEED_XOR = 0x0BAD5EED
 
 
 def count_flops_per_path(scene_spec, w, h, seeds, stride=16):
-    """Algorithmic flops per path, counted by the oracle on every stride-th
-    pixel of every stride-th row (first pass of each)."""
+    """Algorithmic flops per path, counted by the oracle on every pixel of
+    every stride-th row (first pass of each)."""
     import oracle
 
     total, paths = 0, 0
     for y in range(0, h, stride):
-        c, s, f = oracle.render(scene_spec, w, h, 1, seeds, precision=oracle.COUNT, crop=(0, y, w, 1), threads=0)
+        _, _, f = oracle.render(scene_spec, w, h, 1, seeds, precision=oracle.COUNT, crop=(0, y, w, 1), threads=0)
         total += f
         paths += w
     return total / paths, paths
 
 
 def cpu_baseline(scene_spec, w, h, seeds, budget_s=12.0):
-    """Oracle FP32 on rows of the frame, all passes of one sample each, until
-    about `budget_s` of wall time."""
+    """Oracle FP32 on rows of the frame, one sample per pixel, until about
+    `budget_s` of wall time."""
     import oracle
 
     threads = min(os.cpu_count() or 1, 16)
-    paths, t0 = 0, time.perf_counter()
-    y = 0
-    rows = 4
+    paths, t0, y, rows = 0, time.perf_counter(), 0, 4
     while time.perf_counter() - t0 < budget_s and y < h:
-        oracle.render(scene_spec, w, h, 1, seeds, crop=(0, y, w, min(rows, h - y)), threads=threads)
-        paths += w * min(rows, h - y)
-        y += rows
+        n = min(rows, h - y)
+        oracle.render(scene_spec, w, h, 1, seeds, crop=(0, y, w, n), threads=threads)
+        paths += w * n
+        y += n
     dt = time.perf_counter() - t0
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (FP32, OpenMP) on {paths} paths: first {y} rows x 1 spp of the same frame, {dt:.1f} s"}
+            "sample": f"CPU oracle (FP32, OpenMP) on {paths} paths: first {y} rows x 1 spp of the same frame, "
+                      f"{dt:.1f} s"}
+
+
+def gamma_image(canvas):
+    """The image RMSE is taken on: extract, gamma 1/2.2, clamp to [0, 1]; RGB."""
+    c = canvas.astype(np.float64)
+    img = c[..., :3] / c[..., 3:4]
+    return np.clip(np.power(np.maximum(img, 0.0), 1 / 2.2), 0.0, 1.0)
+
+
+def rmse(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)))
+
+
+def time_to_rmse(ctx, wl, buf, chosen, plain_render, args):
+    from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
+    from hypertrace_amd.types import X32_F64
+
+    W, H = wl.width, wl.height
+    t_or = time.perf_counter()
+    half = max(1, args.oracle_spp // 2)
+    halves = []
+    with Context(0, X32_F64) as ctx64:
+        r64 = Render(ctx64, wl.scene)
+        b64 = EntityBuffer(ctx64, wl.scene)
+        for k in range(2):
+            c = Canvas(ctx64, (W, H), 1, wl.seed ^ ORACLE_SEED_XOR ^ (k + 1))
+            done = 0
+            while done < half:
+                n = min(256, half - done)
+                r64.render_passes(b64, c, n)
+                done += n
+            halves.append(c.load_image())
+    oracle_s = time.perf_counter() - t_or
+    ref = gamma_image(halves[0] + halves[1])
+    # noise of the 4096-spp oracle: the two halves differ by twice its variance
+    oracle_noise = rmse(gamma_image(halves[0]), gamma_image(halves[1])) / 2.0
+
+    # spp at which the FP32 image first reaches the target: steps of 16
+    # passes, then pass by pass inside the last step.
+    c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    spp, step, found = 0, 16, None
+    curve = []
+    while spp < 4096:
+        chosen.render_passes(buf, c, step)
+        spp += step
+        e = rmse(gamma_image(c.load_image()), ref)
+        curve.append((spp, round(e, 6)))
+        if e <= args.target_rmse:
+            found = spp
+            break
+    if found is None:
+        return {"reached": False, "curve": curve, "oracle_spp": 2 * half, "oracle_noise_rmse": oracle_noise}
+    c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    start = found - step
+    if start > 0:
+        chosen.render_passes(buf, c, start)
+    spp = start
+    while spp < found:
+        chosen.render_passes(buf, c, 1)
+        spp += 1
+        if rmse(gamma_image(c.load_image()), ref) <= args.target_rmse:
+            break
+    need = spp
+
+    c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    ctx.sync()
+    t0 = time.perf_counter()
+    chosen.render_passes(buf, c, need)
+    ctx.sync()
+    t_chosen = time.perf_counter() - t0
+    res = {"reached": True, "target": args.target_rmse, "spp": need, "chosen_s": t_chosen,
+           "final_rmse": rmse(gamma_image(c.load_image()), ref), "oracle_spp": 2 * half,
+           "oracle_noise_rmse": oracle_noise, "oracle_render_s": oracle_s, "curve": curve,
+           "rmse_definition": "all pixels x RGB of the extracted, gamma-1/2.2, clamped [0,1] image"}
+    if plain_render is not None:
+        c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(need):
+            plain_render.render(buf, c)
+        t_plain = time.perf_counter() - t0
+        res["plain_s"] = t_plain
+        res["speedup"] = t_plain / t_chosen
+    return res
 
 
 def main():
@@ -72,8 +169,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="time only the chosen form")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
+    ap.add_argument("--oracle-spp", type=int, default=4096)
+    ap.add_argument("--target-rmse", type=float, default=0.01)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "
                          "the known-size copy that scales FETCH_SIZE / WRITE_SIZE counter readings")
@@ -92,6 +192,7 @@ def main():
     paths_per_step = W * H * SPP
     spec = wl.scene.spec_text(X32_F32)
     seeds0 = seed_fill(W * H, wl.seed_mode, wl.seed)
+    extra = {}
 
     with Context(0, X32_F32) as ctx:
         buf = EntityBuffer(ctx, wl.scene)
@@ -108,22 +209,13 @@ def main():
                 render.render_passes(buf, canvas, SPP)
             gpu_ms = ctx.timer_end()
             ctx.sync()
-            wall = time.perf_counter() - t0
-            return wall, gpu_ms
+            return time.perf_counter() - t0, gpu_ms
 
-        render = Render(ctx, wl.scene)  # default program: strict IEEE FP32
-        wall, gpu_ms = time_chosen(render, args.steps, args.warmup)
+        chosen = Render(ctx, wl.scene)  # default program: strict IEEE FP32, culled scan
+        wall, gpu_ms = time_chosen(chosen, args.steps, args.warmup)
         ms_per_step = wall * 1e3 / args.steps
         kernel_ms = gpu_ms / args.steps
         mpaths = paths_per_step * args.steps / wall / 1e6
-
-        fast = None
-        if not args.no_plain:
-            render_fast = Render(ctx, wl.scene, fp_options=FAST_FP)
-            fwall, fgpu = time_chosen(render_fast, args.steps, args.warmup)
-            fast = {"kernel": "render_passes, fast FP32 (FMA contraction, reciprocal division, device libm)",
-                    "value": paths_per_step * args.steps / fwall / 1e6, "unit": "Mpaths/s",
-                    "kernel_ms": fgpu / args.steps}
 
         if args.calibrate:
             from hypertrace_amd.processing import Extractor, ImageBuffer
@@ -132,20 +224,30 @@ def main():
             Extractor(ctx).process(big, ImageBuffer(ctx, (4096, 4096)))
             ctx.sync()
 
-        plain = None
+        plain_render = None
         if not args.no_plain:
+            plain_render = Render(ctx, wl.scene, fp_options=NO_CULL)
             canvas_p = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
-            render.render(buf, canvas_p)  # warm
+            plain_render.render(buf, canvas_p)  # warm
             ctx.sync()
             t0 = time.perf_counter()
             ctx.timer_begin()
             for _ in range(SPP):
-                render.render(buf, canvas_p)
+                plain_render.render(buf, canvas_p)
             plain_gpu_ms = ctx.timer_end()
             plain_wall = time.perf_counter() - t0
-            plain = {"kernel": "render (one launch + sync per pass, as the reference runs)",
-                     "value": W * H * SPP / plain_wall / 1e6, "unit": "Mpaths/s",
-                     "ms_per_frame": plain_wall * 1e3, "gpu_ms_per_frame": plain_gpu_ms}
+            extra["plain"] = {"kernel": "render: one launch + sync per pass, element-by-element scan, strict FP32",
+                              "value": W * H * SPP / plain_wall / 1e6, "unit": "Mpaths/s",
+                              "ms_per_frame": plain_wall * 1e3, "gpu_ms_per_frame": plain_gpu_ms}
+            fast = Render(ctx, wl.scene, fp_options=FAST_FP)
+            fwall, fgpu = time_chosen(fast, args.steps, args.warmup)
+            extra["fast_fp"] = {"kernel": "render_passes, culled scan, fast FP32 (FMA contraction, reciprocal "
+                                          "division, device libm; not bit-identical)",
+                                "value": paths_per_step * args.steps / fwall / 1e6, "unit": "Mpaths/s",
+                                "kernel_ms": fgpu / args.steps}
+
+        if not args.no_rmse:
+            extra["time_to_rmse"] = time_to_rmse(ctx, wl, buf, chosen, plain_render, args)
 
     flops_per_path, sample_paths = count_flops_per_path(spec, W, H, seeds0)
     flops_per_launch = flops_per_path * paths_per_step
@@ -154,6 +256,8 @@ def main():
                 "frac": achieved / FP32_VECTOR_PEAK, "frac_nonpacked": achieved / FP32_NONPACKED_PEAK,
                 "kernel": "render_passes", "kernel_ms": kernel_ms,
                 "flops_per_path": flops_per_path, "flops_sample_paths": sample_paths,
+                "flops_note": "algorithmic flops of the reference algorithm; the culled scan executes a fraction "
+                              "of them, so this is an effective rate",
                 "traffic": MEASURED_TRAFFIC_BYTES,
                 "algorithmic_bytes": 40 * W * H + len(wl.scene.to_bytes(X32_F32))}
 
@@ -173,9 +277,8 @@ def main():
         "config": {"workload": "hyperbolic_cells", "width": W, "height": H, "spp": SPP, "hops": 8,
                    "primitives": 192, "kernel": "render_passes"},
         "roofline": roofline,
-        "plain": plain,
-        "fast_fp": fast,
     }
+    out.update(extra)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, W, H, seeds0)
     print(json.dumps(out))

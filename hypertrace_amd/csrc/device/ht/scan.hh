@@ -295,10 +295,33 @@ HD unsigned long long lane_mask_below() {
 }
 
 // Monotonic unsigned key of a float.
+// Key of (distance, element index) whose unsigned order is the reference
+// loop's choice: smaller distance first, then the lower index. FP32: the
+// ordered distance bits above a 32-bit index, exact. FP64: the ordered
+// distance bits with the low 12 mantissa bits replaced by the index (up to
+// 4096 elements), so distances within 2^-40 relative tie and go to the lower
+// index; FP64 programs serve as the accuracy oracle, not as a bit-exact
+// drop-in.
+#ifdef HT_DOUBLE
+HD unsigned long long hit_key(double d, usize i) {
+    unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    unsigned long long k = (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+    return (k & ~0xFFFull) | (unsigned long long)(i & 0xFFF);
+}
+HD usize key_index(unsigned long long k) { return (usize)(k & 0xFFFull); }
+#else
+HD unsigned long long hit_key(float d, usize i);
+HD usize key_index(unsigned long long k) { return (usize)(k & 0xFFFFFFFFull); }
+#endif
 HD uint order_key(float f) {
     uint u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+#ifndef HT_DOUBLE
+HD unsigned long long hit_key(float d, usize i) {
+    return ((unsigned long long)order_key(d) << 32) | (unsigned long long)i;
+}
+#endif
 
 // ------------------------------------------------------------ the scan
 // Top-level object of a scene: the culled scan for a vector of cullable
@@ -314,10 +337,10 @@ template <class T> struct TopScan {
 template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     typedef ObjectVec<G, E> OV;
     typedef typename OV::V V;
-#if defined(HT_DOUBLE) || defined(HT_NO_CULL)
-    // FP64: the packed (distance, index) key is FP32. HT_NO_CULL: the plain
-    // scan on request (for comparisons).
-    static constexpr bool CULL = false;
+#if defined(HT_NO_CULL)
+    static constexpr bool CULL = false;  // the plain scan on request (for comparisons)
+#elif defined(HT_DOUBLE)
+    static constexpr bool CULL = Cullable<E>::value && (BOUNDS_CAP <= 4096);
 #else
     static constexpr bool CULL = Cullable<E>::value;
 #endif
@@ -360,8 +383,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             Ray<G> local;
             real dist = element_test(self, i, ws->ray[L], ws->orig_hasher[L], ws->prev_hash[L], &cc, &local);
             if (dist > -EPS) {
-                unsigned long long key = ((unsigned long long)order_key((float)dist) << 32) | (unsigned long long)i;
-                atomicMin(&ws->best[L], key);
+                atomicMin(&ws->best[L], hit_key(dist, i));
             }
         }
     }
@@ -415,7 +437,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 ctx->hasher.state = orig;
                 return -R1;
             }
-            usize i = (usize)(best & 0xffffffffull);
+            usize i = key_index(best);
             typename E::Cache cc;
             Ray<G> local;
             real dist = element_test(self, i, light->ray, orig, ctx->prev_hash, &cc, &local);
