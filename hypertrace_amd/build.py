@@ -49,15 +49,17 @@ def standard_scenes():
         scenes.euclidean_spheres().scene,
         scenes.euclidean_spheres(unit_spheres=True).scene,
         scenes.hyperbolic_cells().scene,
+        scenes.reference_euclidean_example().scene,
+        scenes.reference_hyperbolic_example().scene,
     ]
 
 
 def precompile_programs(verbose: bool = False) -> None:
-    from .processing import FAST_FP, precompile
+    from .processing import FAST_FP, NO_CULL_FP, precompile
     from .types import X32_F32, X32_F64
 
     for scene in standard_scenes():
-        for cfg, fp in ((X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP)):
+        for cfg, fp in ((X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP), (X32_F32, NO_CULL_FP)):
             t = time.time()
             hit = precompile(scene, cfg, fp_options=fp)
             if verbose:

@@ -14,7 +14,7 @@ import glob
 import os
 from typing import Dict, List, Tuple
 
-from .objects import Entity, SceneImpl, _ChoiceBase, _Component, _VecBase, Mapped
+from .objects import Entity, Mapped, SceneImpl, _ChoiceBase, _Component, _Tiled, _VecBase
 from .types import Config
 
 DEVICE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "device")
@@ -39,6 +39,8 @@ def _children(e: Entity) -> List[Entity]:
         return list(e.variants)
     if isinstance(e, Mapped):
         return [e.map, e.inner]
+    if isinstance(e, _Tiled):
+        return list(e.materials) + [e.border_material]
     kids = []
     for name in ("inner", "shape", "material"):
         if hasattr(e, name) and isinstance(getattr(e, name), Entity):

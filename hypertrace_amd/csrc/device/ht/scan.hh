@@ -25,6 +25,7 @@
 //    get the hit state, and maps it back to the parent frame once.
 #pragma once
 #include "entities.hh"
+#include "tiled.hh"
 
 #ifndef HT_BOUNDS_CAP
 #define HT_BOUNDS_CAP 256
@@ -76,14 +77,33 @@ template <class G, class... Vs> struct PrimKind<ShapeChoice<G, Vs...>> {
     static HD double radius(const u8 *s) { return radius_at<0, Vs...>(C::index(s), C::data(s)); }
 };
 
-// Elements the culled scan handles: Mapped<G, Covered<G, S, Mat>, M>.
+// Primitive of an object entity, read from its bytes.
+template <class TI> struct ObjPrim {
+    static constexpr bool OK = false;
+    static HD int kind(const u8 *) { return 0; }
+    static HD double radius(const u8 *) { return 0.0; }
+};
+template <class G, class S, class Mat> struct ObjPrim<Covered<G, S, Mat>> {
+    static constexpr bool OK = true;
+    typedef Covered<G, S, Mat> C;
+    static HD int kind(const u8 *o) { return PrimKind<S>::kind(o + C::L::offset(1)); }
+    static HD double radius(const u8 *o) { return PrimKind<S>::radius(o + C::L::offset(1)); }
+};
+template <class Mat, int TL, int N> struct ObjPrim<TiledPlane<Mat, TL, N>> {
+    static constexpr bool OK = true;
+    static HD int kind(const u8 *) { return 4; }
+    static HD double radius(const u8 *) { return 1.0; }
+};
+template <class Mat, int TL, int N> struct ObjPrim<TiledHorosphere<Mat, TL, N>> {
+    static constexpr bool OK = true;
+    static HD int kind(const u8 *) { return 5; }
+    static HD double radius(const u8 *) { return 1.0; }
+};
+
+// Elements the culled scan handles: a mapped object over one primitive.
 template <class T> struct Cullable { static constexpr bool value = false; };
-template <class G, class S, class Mat, class M> struct Cullable<Mapped<G, Covered<G, S, Mat>, M>> {
-    static constexpr bool value = true;
-    typedef Covered<G, S, Mat> CoveredT;
-    static HD const u8 *shape(const u8 *elem) {
-        return Mapped<G, CoveredT, M>::inner(elem) + CoveredT::L::offset(1);
-    }
+template <class G, class TI, class M> struct Cullable<Mapped<G, TI, M>> {
+    static constexpr bool value = ObjPrim<TI>::OK;
 };
 
 // ------------------------------------------------------------ world bounds
@@ -153,12 +173,9 @@ template <> struct WorldBound<Shift3> {
 };
 
 template <class T, class M> HD Bound element_bound(const u8 *elem) {
-    typedef Cullable<T> CT;
-    typedef typename CT::CoveredT Cov;
-    const u8 *shape = CT::shape(elem);
-    int kind = PrimKind<typename Cov::ShapeT>::kind(shape);
-    M m = T::map(elem);
-    return WorldBound<M>::of(m, kind, PrimKind<typename Cov::ShapeT>::radius(shape));
+    typedef typename T::Inner TI;
+    const u8 *inner = T::inner(elem);
+    return WorldBound<M>::of(T::map(elem), ObjPrim<TI>::kind(inner), ObjPrim<TI>::radius(inner));
 }
 
 // ------------------------------------------------------------ ray cull tests

@@ -674,6 +674,85 @@ class Covered(Entity):
         return ["covered"] + self.shape.spec(geo, cfg) + self.material.spec(geo, cfg)
 
 
+class _Array(Entity):
+    """[T; N]: elements back to back, align of T (types/src/containers/array.rs)."""
+
+    def __init__(self, items):
+        self.items = list(items)
+
+    def align(self, cfg):
+        return self.items[0].align(cfg)
+
+    def static_size(self, cfg):
+        return len(self.items) * self.items[0].static_size(cfg)
+
+    def size(self, cfg):
+        return self.static_size(cfg)
+
+    def store(self, cfg, w):
+        for it in self.items:
+            it.store(cfg, w)
+
+
+PLANE_TILINGS = {"uniform": 0, "pentagonal": 1, "pentastar": 2}
+HOROSPHERE_TILINGS = {"uniform": 0, "square": 1, "hexagonal": 2}
+
+
+class _Tiled(Entity):
+    """{materials: [M; N], border_material: M, cell_size, border_width}
+    (objects/src/object/tiled_plane.rs:35-42, tiled_horosphere.rs:35-42)."""
+
+    TILINGS: dict = {}
+    CPP = ""
+    TOKEN = ""
+
+    def __init__(self, materials, cell_size, border_width, border_material, tiling):
+        self.materials = list(materials)
+        if tiling not in self.TILINGS:
+            raise ValueError(f"unknown tiling {tiling!r}")
+        self.tiling = tiling
+        self.cell_size = float(cell_size)
+        self.border_width = float(border_width)
+        self.border_material = border_material
+        t = border_material.cpp(HY)
+        for m in self.materials:
+            if m.cpp(HY) != t:
+                raise TypeError("tile materials must have one type")
+
+    def fields(self):
+        return [_Array(self.materials), self.border_material, _Real(self.cell_size), _Real(self.border_width)]
+
+    def cpp(self, geo):
+        assert geo == HY
+        return (f"ht::{self.CPP}<{self.border_material.cpp(geo)}, {self.TILINGS[self.tiling]}, "
+                f"{len(self.materials)}>")
+
+    def spec(self, geo, cfg):
+        out = [self.TOKEN, str(self.TILINGS[self.tiling]), str(len(self.materials)), _hr(self.cell_size, cfg),
+               _hr(self.border_width, cfg)]
+        for m in self.materials:
+            out += m.spec(geo, cfg)
+        return out + self.border_material.spec(geo, cfg)
+
+
+class TiledPlane(_Tiled):
+    """Geodesic plane tiled uniformly or by the {5,4} pentagonal /
+    pentastar tiling (object/hy/tiled_plane/impl.inl)."""
+
+    TILINGS = PLANE_TILINGS
+    CPP = "TiledPlane"
+    TOKEN = "tiledplane"
+
+
+class TiledHorosphere(_Tiled):
+    """Horosphere tiled uniformly, by squares or by hexagons
+    (object/hy/tiled_horosphere/impl.inl)."""
+
+    TILINGS = HOROSPHERE_TILINGS
+    CPP = "TiledHorosphere"
+    TOKEN = "tiledhoro"
+
+
 # --------------------------------------------------------------------- views
 
 class PointView(Entity):

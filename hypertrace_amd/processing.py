@@ -206,6 +206,7 @@ class Program:
 # Program FP modes (see include/hypertrace_hip.h, ht_prog_build_ex).
 STRICT_FP = "-ffp-contract=off -DHT_STRICT_FP=1"  # the default
 FAST_FP = "-fno-hip-fp32-correctly-rounded-divide-sqrt"
+NO_CULL_FP = STRICT_FP + " -DHT_NO_CULL=1"  # strict, element-by-element scan
 
 
 def precompile(scene: SceneImpl, config: Config, name: str = "render.hip", fp_options: Optional[str] = None) -> bool:

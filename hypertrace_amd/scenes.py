@@ -143,3 +143,79 @@ def hyperbolic_cells(width: int = 640, height: int = 480, spp: int = 64, cells: 
     bg = ConstBg((1.0, 1.0, 1.0))
     scene = SceneImpl(HY, view, ObjectVec(objs), bg, hops)
     return Workload("hyperbolic_cells", scene, width, height, spp, 1, MASTER_SEED ^ 2)
+
+
+def _unpack_color(rgb: int):
+    """0xRRGGBB -> linear colour, as main/examples/hy.rs:27-35 (gamma 2.2)."""
+    return [(((rgb >> s) & 0xFF) / 255.0) ** 2.2 for s in (16, 8, 0)]
+
+
+def reference_euclidean_example(width: int = 800, height: int = 600, spp: int = 1) -> Workload:
+    """The reference's Euclidean example scene (main/examples/eu.rs:30-136):
+    a sphere (refractive / specular mixture), a cube and a plane, gradient sky,
+    camera at its initial controller position (0, 0.5, 2), 4 hops."""
+    from .objects import Cube, Refractive
+
+    choice = [Plane(), Sphere(), Cube()]
+
+    def mixture(d, s, r):
+        return Mixture([(Colored(Lambertian(), d[0]), d[1]), (Specular(), s), (Colored(Refractive(r[0]), r[1]), r[2])])
+
+    objs = [
+        Mapped(Covered(ShapeChoice(choice, 1), mixture(([1.0, 0.2, 0.2], 0.0), 0.1, (1.2, [1.0, 1.0, 0.2], 0.9))),
+               Shift3((0.0, 1.0, 0.0))),
+        Mapped(Covered(ShapeChoice(choice, 2), mixture(([0.2, 0.8, 0.8], 1.0), 0.0, (1.0, [1.0, 1.0, 1.0], 0.0))),
+               Shift3((0.0, -1.0, 0.0))),
+        Mapped(Covered(ShapeChoice(choice, 0), mixture(([1.0, 1.0, 1.0], 0.9), 0.1, (1.0, [1.0, 1.0, 1.0], 0.0))),
+               Shift3((0.0, 0.0, -1.0))),
+    ]
+    view = Mapped(PointView(1.0), Homogenous3((0.0, 0.5, 2.0), (1, 0, 0, 0)))
+    bg = GradBg((0.0, 1.0, 0.0), [(1.0, 1.0, 1.0), (0.0, 0.0, 0.0)], 2.4)
+    scene = SceneImpl(EU, view, ObjectVec(objs), bg, 4)
+    return Workload("reference_euclidean_example", scene, width, height, spp, 0, 0xDEADBEEFDEADBEEF)
+
+
+def reference_hyperbolic_example(width: int = 800, height: int = 600, spp: int = 1) -> Workload:
+    """The reference's hyperbolic example scene (main/examples/hy.rs:37-183):
+    two tiled horospheres (hexagonal, square) and two tiled planes
+    (pentastar, pentagonal) with emissive borders, constant sky, camera at its
+    initial controller map rotate_z(5pi/6) rotate_x(2pi/5) shift_z(2), 3 hops.
+    The reference builds its maps with vecmat/ccgeom; here they are built in
+    FP64 from the generator formulas of geometry/hyperbolic.cc:35-62."""
+    from .objects import ObjectChoice, TiledHorosphere, TiledPlane, Transparent, hy_xrotate
+
+    def make_material(diffuse, specularity, transparency, emission=None):
+        mix = Mixture([(Colored(Lambertian(), diffuse), 1.0 - specularity - transparency), (Specular(), specularity),
+                       (Transparent(), transparency)])
+        return Emissive(mix, emission if emission is not None else [0.0, 0.0, 0.0])
+
+    border = make_material(_unpack_color(0xE4E4E4), 0.0, 0.0, [1.0, 1.0, 1.0])
+    proto_m = make_material([0.5, 0.5, 0.5], 0.0, 0.0)
+    protos = [TiledPlane([proto_m] * 2, 0.0, 0.0, proto_m, "pentastar"),
+              TiledPlane([proto_m] * 2, 0.0, 0.0, proto_m, "pentagonal"),
+              TiledHorosphere([proto_m] * 3, 1.0, 0.0, proto_m, "hexagonal"),
+              TiledHorosphere([proto_m] * 4, 1.0, 0.0, proto_m, "square")]
+
+    def choice(index, value):
+        variants = list(protos)
+        variants[index] = value
+        return ObjectChoice(variants, index)
+
+    objs = [
+        Mapped(choice(2, TiledHorosphere([make_material(_unpack_color(c), 0.1, 0.1) for c in
+                                          (0xFE0000, 0xFFAA01, 0x35ADAE)], 0.5, 0.02, border, "hexagonal")),
+               Moebius.identity()),
+        Mapped(choice(3, TiledHorosphere([make_material(_unpack_color(c), 0.1, 0.1) for c in
+                                          (0xFE7401, 0xFE0000, 0xFFAA01, 0xFED601)], 0.5, 0.02, border, "square")),
+               Moebius([[1.0, math.sqrt(2.0)], [0.0, 1.0]]).chain(hy_xrotate(math.pi))),
+        Mapped(choice(0, TiledPlane([make_material(_unpack_color(c), 0.1, 0.0) for c in (0xFE7401, 0x35ADAE)],
+                                    float("nan"), 0.01, border, "pentastar")),
+               Moebius.identity()),
+        Mapped(choice(1, TiledPlane([make_material(_unpack_color(c), 0.1, 0.0) for c in (0xFE0000, 0xFED601)],
+                                    float("nan"), 0.02, border, "pentagonal")),
+               Moebius([[1.0, 2.0j], [0.0, 1.0]])),
+    ]
+    cam = hy_zrotate(5.0 * math.pi / 6.0).chain(hy_xrotate(2.0 * math.pi / 5.0)).chain(hy_zshift(2.0))
+    view = Mapped(PointView(1.0), cam)
+    scene = SceneImpl(HY, view, ObjectVec(objs), ConstBg(_unpack_color(0xEEEEEE)), 3)
+    return Workload("reference_hyperbolic_example", scene, width, height, spp, 0, 0xDEADBEEFDEADBEEF)

@@ -77,6 +77,15 @@ inline Cnt fcos(Cnt x) { ++g_flops; return Cnt(std::cos(x.v)); }
 inline float fsin(float x) { return (float)std::sin((double)x); }
 inline double fsin(double x) { return std::sin(x); }
 inline Cnt fsin(Cnt x) { ++g_flops; return Cnt(std::sin(x.v)); }
+inline float fcosh(float x) { return (float)std::cosh((double)x); }
+inline double fcosh(double x) { return std::cosh(x); }
+inline Cnt fcosh(Cnt x) { ++g_flops; return Cnt((float)std::cosh((double)x.v)); }
+inline float fsinh(float x) { return (float)std::sinh((double)x); }
+inline double fsinh(double x) { return std::sinh(x); }
+inline Cnt fsinh(Cnt x) { ++g_flops; return Cnt((float)std::sinh((double)x.v)); }
+inline float ffloor(float x) { return std::floor(x); }
+inline double ffloor(double x) { return std::floor(x); }
+inline Cnt ffloor(Cnt x) { return Cnt(std::floor(x.v)); }
 inline float fpow(float a, float b) { return (float)std::pow((double)a, (double)b); }
 inline double fpow(double a, double b) { return std::pow(a, b); }
 inline Cnt fpow(Cnt a, Cnt b) { ++g_flops; return Cnt(std::pow(a.v, b.v)); }
@@ -95,14 +104,17 @@ template <class R> struct Consts;
 template <> struct Consts<float> {
     static float eps() { return 1e-6f; }
     static float pi() { return 3.14159265358979f; }
+    static float fract_max() { return 0x1.fffffep-1f; }
 };
 template <> struct Consts<double> {
     static double eps() { return 1e-12; }
     static double pi() { return 3.141592653589793; }
+    static double fract_max() { return 0x1.fffffffffffffp-1; }
 };
 template <> struct Consts<Cnt> {
     static Cnt eps() { return Cnt(1e-6f); }
     static Cnt pi() { return Cnt(3.14159265358979f); }
+    static Cnt fract_max() { return Cnt(0x1.fffffep-1f); }
 };
 
 // ------------------------------------------------------------ vectors
@@ -645,6 +657,149 @@ template <class R, class F, class G> struct Covered : Object<R, F, G> {
     }
 };
 
+// Hyperbolic generators in R, geometry/hyperbolic.cc:40-53 (c*C1 = (c*1, c*0)).
+template <class R> C22<R> hy_xshift_m(R l) {
+    R l2 = l / R(2);
+    R c = fcosh(l2), s = fsinh(l2);
+    return {{c * R(1), c * R(0)}, {s * R(1), s * R(0)}, {s * R(1), s * R(0)}, {c * R(1), c * R(0)}};
+}
+template <class R> C22<R> hy_zrotate_m(R phi) {
+    R c = fcos(phi / R(2)), s = fsin(phi / R(2));
+    return {{c, s}, {R(0), R(0)}, {R(0), R(0)}, {c, -s}};
+}
+template <class R> R fract_ocl(R x, R &ip) {  // OpenCL fract: kept below 1
+    ip = ffloor(x);
+    return fmin_(x - ip, Consts<R>::fract_max());
+}
+inline int rem_i(int a, int b) { return ((a % b) + b) % b; }
+
+// Tiled plane: object/hy/tiled_plane/impl.inl:22-148.
+template <class R, class F> struct TiledPlaneObj : Object<R, F, GeoHy<R>> {
+    int tiling = 0;
+    R cell = R(0), border = R(0);
+    std::vector<std::unique_ptr<Material<R, F>>> mats;
+    std::unique_ptr<Material<R, F>> border_mat;
+    R detect(Ctx &ctx, Cache<R, GeoHy<R>> &cache, Light<R, F, GeoHy<R>> &light) const override {
+        return PlaneHy<R, F>().detect(ctx, cache.normal, light);
+    }
+    const Material<R, F> *select(V4<R> p) const {
+        if (tiling != 1 && tiling != 2) return mats[0].get();
+        const R PI = Consts<R>::pi();
+        bool w = false;
+        R Q = fsqrt(fcos(PI / R(4) + PI / R(5)) / fcos(PI / R(4) - PI / R(5)));
+        R T = fsqrt(fcos(PI / R(4) + PI / R(5)) * fcos(PI / R(4) - PI / R(5)));
+        R S = (fcos(PI / R(4)) - fsin(PI / R(5))) / T;
+        R L = T / fcos(PI / R(4));
+        R K = L * (R(2) * fcos(PI / R(5)) - R(1) / fcos(PI / R(5)));
+        Q = flog((R(1) + Q) / (R(1) - Q));
+        S = flog((R(1) + S) / (R(1) - S));
+        if (p.x < R(0)) {
+            p.x = -p.x;
+            w = !w;
+        }
+        if (p.y < R(0)) {
+            p.y = -p.y;
+            w = !w;
+        }
+        p = MoMap<R>(c22_dot(hy_xshift_m(-Q), hy_zrotate_m(-PI / R(4)))).apply_pos(p);
+        bool e = false;
+        for (int j = 0; j < 5; ++j) {
+            bool a[3] = {false, false, false};
+            for (int i = 0; i < 3 - (int)e; ++i) {
+                R o = R(2) * PI * R(i - 1) / R(5);
+                a[i] = (fcos(o) * p.x + fsin(o) * p.y) < L;
+            }
+            a[2] = a[2] || e;
+            int s = (int)a[0] + (int)a[1] + (int)a[2];
+            if (s == 3) break;
+            if (s == 2) {
+                int i = (int)(!a[1]) + 2 * (int)(!a[2]);
+                R o = R(2) * PI * R(i - 1) / R(5);
+                p = MoMap<R>(c22_dot(hy_zrotate_m(-PI / R(5)), c22_dot(hy_xshift_m(R(-2) * S), hy_zrotate_m(-o))))
+                        .apply_pos(p);
+                e = true;
+                w = !w;
+            } else {
+                int i = (int)a[0];
+                R o = PI * R(2 * i - 1) / R(5);
+                p = MoMap<R>(c22_dot(hy_xshift_m(R(-2) * Q), hy_zrotate_m(-o))).apply_pos(p);
+                e = false;
+            }
+        }
+        bool bh = false;
+        for (int i = 0; i < 5; ++i) {
+            R o = R(2) * PI * R(i) / R(5);
+            R dd = fcos(o) * p.x + fsin(o) * p.y;
+            bh = bh || (dd > (L - border * p.z));
+            if (tiling == 2) {
+                R ps = K + dd;
+                w = w ^ (ps < R(0));
+                bh = bh || (fabs_(ps) < border * p.z);
+            }
+        }
+        if (!bh) return (w || mats.size() < 2) ? mats[0].get() : mats[1].get();
+        return border_mat.get();
+    }
+    bool interact(Ctx &ctx, Cache<R, GeoHy<R>> &cache, Light<R, F, GeoHy<R>> &light, C3<F> &e) const override {
+        const Material<R, F> *m = select(light.ray.start);
+        LightLocal<R, F> ll = to_local(light);
+        bool ret = m->interact(ctx, GeoHy<R>::to_local(light.ray.start, cache.normal), ll, e);
+        from_local(light, ll);
+        return ret;
+    }
+};
+
+// Tiled horosphere: object/hy/tiled_horosphere/impl.inl:29-110.
+template <class R, class F> struct TiledHoroObj : Object<R, F, GeoHy<R>> {
+    int tiling = 0;
+    R cell = R(0), border = R(0);
+    std::vector<std::unique_ptr<Material<R, F>>> mats;
+    std::unique_ptr<Material<R, F>> border_mat;
+    R detect(Ctx &ctx, Cache<R, GeoHy<R>> &cache, Light<R, F, GeoHy<R>> &light) const override {
+        return HorosphereHy<R, F>().detect(ctx, cache.normal, light);
+    }
+    const Material<R, F> *select(V4<R> p) const {
+        int n = (int)mats.size();
+        int material_no = 0;
+        bool is_border = false;
+        R br = border / cell;
+        if (tiling == 1) {
+            R kx, ky;
+            R fx = fract_ocl(p.x / cell, kx), fy = fract_ocl(p.y / cell, ky);
+            if (fx < br || fx > R(1) - br || fy < br || fy > R(1) - br) {
+                is_border = true;
+            } else {
+                int hx = rem_i((int)tofloat(kx), 2), hy = rem_i((int)tofloat(ky), 2);
+                material_no = 3 * hy - 2 * hx * hy + hx;
+            }
+        } else if (tiling == 2) {
+            R s3 = fsqrt(R(3));
+            R bxx = R(2) / s3, bxy = R(0), byx = R(-1) / s3, byy = R(1);
+            R s = cell * s3 / R(2);
+            R hxv = (bxx * p.x + bxy * p.y) / s, hyv = (byx * p.x + byy * p.y) / s;
+            int hx = (int)tofloat(ffloor((ffloor(hxv) - ffloor(hyv)) / R(3)));
+            int hy = (int)tofloat(ffloor((ffloor(hxv + hyv) - R(hx)) / R(2)));
+            R sx = R(hx) * R(2) + R(hy) * R(1), sy = R(hx) * R(-1) + R(hy) * R(1);
+            hxv = hxv - sx;
+            hyv = hyv - sy;
+            if (fabs_(hxv - R(1)) > R(1) - br || fabs_(hyv) > R(1) - br || fabs_(hxv + hyv - R(1)) > R(1) - br) {
+                is_border = true;
+            } else {
+                material_no = 2 * hx + hy;
+            }
+        }
+        if (is_border) return border_mat.get();
+        return mats[rem_i(material_no, n)].get();
+    }
+    bool interact(Ctx &ctx, Cache<R, GeoHy<R>> &cache, Light<R, F, GeoHy<R>> &light, C3<F> &e) const override {
+        const Material<R, F> *m = select(light.ray.start);
+        LightLocal<R, F> ll = to_local(light);
+        bool ret = m->interact(ctx, GeoHy<R>::to_local(light.ray.start, cache.normal), ll, e);
+        from_local(light, ll);
+        return ret;
+    }
+};
+
 // object/mapped.inl:24-51
 template <class R, class F, class G> struct MappedObject : Object<R, F, G> {
     std::unique_ptr<Map<R, G>> map, inv;
@@ -806,6 +961,7 @@ struct Tokens {
 };
 
 template <class R, class F, class G> struct Parser;
+template <class R, class F> struct ShapeParse;
 
 template <class R, class F> struct GeoParse;
 template <class R, class F> struct GeoParse {
@@ -892,6 +1048,11 @@ template <class R, class F, class G> struct Parser {
             o->inner = object(level);
             return std::unique_ptr<Object<R, F, G>>(o);
         }
+        if (k == "tiledplane" || k == "tiledhoro") {
+            auto o = ShapeParse<R, F>::tiled(*this, k);
+            if (!o) throw std::runtime_error(k + " needs the hyperbolic geometry");
+            return o;
+        }
         if (k == "objvec") {
             auto o = new VecObject<R, F, G>();
             o->level = level;
@@ -934,6 +1095,22 @@ template <class R, class F> struct ShapeParse {
         (void)p;
         if (k == "plane") return std::unique_ptr<Shape<R, F, GeoHy<R>>>(new PlaneHy<R, F>());
         if (k == "horosphere") return std::unique_ptr<Shape<R, F, GeoHy<R>>>(new HorosphereHy<R, F>());
+        return nullptr;
+    }
+    template <class T> static std::unique_ptr<Object<R, F, GeoHy<R>>> tiled_fill(Parser<R, F, GeoHy<R>> &p, T *o) {
+        o->tiling = (int)p.tk.integer();
+        long n = p.tk.integer();
+        o->cell = R(p.tk.num());
+        o->border = R(p.tk.num());
+        for (long i = 0; i < n; ++i) o->mats.push_back(p.material());
+        o->border_mat = p.material();
+        return std::unique_ptr<Object<R, F, GeoHy<R>>>(o);
+    }
+    static std::unique_ptr<Object<R, F, GeoHy<R>>> tiled(Parser<R, F, GeoHy<R>> &p, const std::string &k) {
+        if (k == "tiledplane") return tiled_fill(p, new TiledPlaneObj<R, F>());
+        return tiled_fill(p, new TiledHoroObj<R, F>());
+    }
+    static std::unique_ptr<Object<R, F, GeoEu<R>>> tiled(Parser<R, F, GeoEu<R>> &, const std::string &) {
         return nullptr;
     }
     static std::unique_ptr<Map<R, GeoEu<R>>> map(Parser<R, F, GeoEu<R>> &p) { return GeoParse<R, F>::map_eu(p.tk); }

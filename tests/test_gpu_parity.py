@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from hypertrace_amd import scenes
-from hypertrace_amd.processing import (STRICT_FP, Canvas, Context, EntityBuffer, Extractor, GammaFilter, ImageBuffer, Packer,
+from hypertrace_amd.processing import (NO_CULL_FP, Canvas, Context, EntityBuffer, Extractor, GammaFilter, ImageBuffer, Packer,
                                        Pipeline, Render, seed_fill)
 from hypertrace_amd.types import X32_F32, X32_F64
 
@@ -53,6 +53,8 @@ WORKLOADS = {
     "eu_ball": lambda: scenes.euclidean_spheres(96, 72),
     "eu_unit": lambda: scenes.euclidean_spheres(96, 72, unit_spheres=True),
     "hy_cells": lambda: scenes.hyperbolic_cells(96, 72),
+    "eu_example": lambda: scenes.reference_euclidean_example(96, 72),
+    "hy_example": lambda: scenes.reference_hyperbolic_example(96, 72),
 }
 
 
@@ -70,7 +72,7 @@ def test_single_pass_parity_f32(ctx, name):
     assert (got_seeds[ok] == ref_seeds[ok]).mean() >= 0.999
 
 
-@pytest.mark.parametrize("name", ["eu_unit", "hy_cells"])
+@pytest.mark.parametrize("name", ["eu_unit", "hy_cells", "eu_example", "hy_example"])
 def test_single_pass_parity_f64(ctx64, name):
     wl = WORKLOADS[name]()
     print(f"case {name} f64", flush=True)
@@ -157,11 +159,11 @@ def test_pipeline_fused_post_equals_separate(ctx):
     assert np.all(a[..., 3] == 255)
 
 
-NO_CULL = STRICT_FP + " -DHT_NO_CULL=1"
+NO_CULL = NO_CULL_FP
 
 
 @pytest.mark.parametrize("name,full", [("eu_ball", False), ("eu_unit", False), ("hy_cells", False),
-                                       ("hy_cells", True)])
+                                       ("hy_cells", True), ("eu_example", False), ("hy_example", False)])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full):
     """The culled, compacted nearest-hit scan must give bit-identical canvas
     and seeds to the element-by-element scan."""
