@@ -16,10 +16,11 @@ region. Beside it:
   chosen form vs plain kernel. Both forms produce bit-identical canvases for
   the same number of passes, so the spp needed is found once and each form is
   then timed over exactly that many passes;
-* roofline: the dominant kernel's algorithmic FP32 flops (counted by the CPU
-  oracle's counting scalar on a sample of the same paths: the work of the
-  reference algorithm, which the culled scan skips most of) per second of its
-  HIP-event time, against the MI355X FP32 vector peak;
+* roofline: the dominant kernel's algorithmic flops (counted by the CPU
+  oracle's counting scalars on a sample of the same paths, for the algorithm
+  the kernel runs: FP32 exact tests of the cull's candidates plus the FP64
+  cull tests) per second of its HIP-event time, against the MI355X FP32
+  vector peak; the plain algorithm's count is reported beside it;
 * cpu_baseline: the CPU oracle (a restatement of the reference kernel) on
   rows of the same frame, on the host's cores.
 
@@ -50,34 +51,45 @@ ORACLE_SEED_XOR = 0x0BAD5EED
 
 
 def count_flops_per_path(scene_spec, w, h, seeds, stride=16):
-    """Algorithmic flops per path, counted by the oracle on every pixel of
-    every stride-th row (first pass of each)."""
+    """Flops per path counted by the oracle's counting scalars on every pixel
+    of every stride-th row (first pass of each): the culled scan's algorithm
+    (FP32 exact tests of the candidates + FP64 cull tests) and the plain
+    reference algorithm (every element's exact test)."""
     import oracle
 
-    total, paths = 0, 0
+    culled32 = culled64 = plain32 = 0
+    paths = 0
     for y in range(0, h, stride):
-        _, _, f = oracle.render(scene_spec, w, h, 1, seeds, precision=oracle.COUNT, crop=(0, y, w, 1), threads=0)
-        total += f
+        _, _, c32, c64 = oracle.count_flops(scene_spec, w, h, 1, seeds, True, crop=(0, y, w, 1))
+        _, _, p32, _ = oracle.count_flops(scene_spec, w, h, 1, seeds, False, crop=(0, y, w, 1))
+        culled32 += c32
+        culled64 += c64
+        plain32 += p32
         paths += w
-    return total / paths, paths
+    return {"fp32": culled32 / paths, "fp64": culled64 / paths, "plain_fp32": plain32 / paths, "paths": paths}
 
 
 def cpu_baseline(scene_spec, w, h, seeds, budget_s=12.0):
-    """Oracle FP32 on rows of the frame, one sample per pixel, until about
+    """Oracle FP32 on bands of rows of the frame, one sample per pixel per
+    pass, pass after pass (seeds advancing as on the GPU), until about
     `budget_s` of wall time."""
     import oracle
 
     threads = min(os.cpu_count() or 1, 16)
-    paths, t0, y, rows = 0, time.perf_counter(), 0, 4
-    while time.perf_counter() - t0 < budget_s and y < h:
+    rows = 16
+    canvas = np.zeros((h, w, 4), np.float32)
+    seeds = np.array(seeds, dtype=np.uint32).reshape(h, w)
+    paths, t0, bands, y = 0, time.perf_counter(), 0, 0
+    while time.perf_counter() - t0 < budget_s:
         n = min(rows, h - y)
-        oracle.render(scene_spec, w, h, 1, seeds, crop=(0, y, w, n), threads=threads)
+        canvas, seeds, _ = oracle.render(scene_spec, w, h, 1, seeds, canvas, crop=(0, y, w, n), threads=threads)
         paths += w * n
-        y += n
+        bands += 1
+        y = (y + n) % h
     dt = time.perf_counter() - t0
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (FP32, OpenMP) on {paths} paths: first {y} rows x 1 spp of the same frame, "
-                      f"{dt:.1f} s"}
+            "sample": f"CPU oracle (FP32, OpenMP, {threads} threads) on {paths} paths of the same frame: "
+                      f"{bands} bands of {rows} rows x 1 spp, cycling over the frame, {dt:.1f} s"}
 
 
 def gamma_image(canvas):
@@ -249,15 +261,19 @@ def main():
         if not args.no_rmse:
             extra["time_to_rmse"] = time_to_rmse(ctx, wl, buf, chosen, plain_render, args)
 
-    flops_per_path, sample_paths = count_flops_per_path(spec, W, H, seeds0)
-    flops_per_launch = flops_per_path * paths_per_step
-    achieved = flops_per_launch / (kernel_ms * 1e-3) / 1e12
+    fl = count_flops_per_path(spec, W, H, seeds0)
+    flops_per_path = fl["fp32"] + fl["fp64"]
+    achieved = flops_per_path * paths_per_step / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK, "unit": "TFLOP/s",
                 "frac": achieved / FP32_VECTOR_PEAK, "frac_nonpacked": achieved / FP32_NONPACKED_PEAK,
                 "kernel": "render_passes", "kernel_ms": kernel_ms,
-                "flops_per_path": flops_per_path, "flops_sample_paths": sample_paths,
-                "flops_note": "algorithmic flops of the reference algorithm; the culled scan executes a fraction "
-                              "of them, so this is an effective rate",
+                "flops_per_path": flops_per_path, "fp32_flops_per_path": fl["fp32"],
+                "fp64_flops_per_path": fl["fp64"], "flops_sample_paths": fl["paths"],
+                "flops_note": "algorithmic flops of the algorithm the kernel runs (FP32 exact tests of the "
+                              "candidates, FP64 cull tests; FP64 and non-packed FP32 issue at the same rate), "
+                              "counted by the CPU oracle on the same paths",
+                "plain_flops_per_path": fl["plain_fp32"],
+                "plain_equivalent_tflops": fl["plain_fp32"] * paths_per_step / (kernel_ms * 1e-3) / 1e12,
                 "traffic": MEASURED_TRAFFIC_BYTES,
                 "algorithmic_bytes": 40 * W * H + len(wl.scene.to_bytes(X32_F32))}
 

@@ -29,6 +29,10 @@ def lib():
         l.oracle_render.restype = c_int
         l.oracle_render.argtypes = [c_char_p, c_int, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p, c_uint32,
                                     c_uint32, c_uint32, c_uint32, c_int, POINTER(c_uint64)]
+        l.oracle_count_flops.restype = c_int
+        l.oracle_count_flops.argtypes = [c_char_p, c_int, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p,
+                                         c_uint32, c_uint32, c_uint32, c_uint32, c_int, POINTER(c_uint64),
+                                         POINTER(c_uint64)]
         l.oracle_lcg_next.restype = c_uint32
         l.oracle_lcg_next.argtypes = [POINTER(c_uint32)]
         l.oracle_uniform_f32.restype = c_float
@@ -66,6 +70,22 @@ def render(spec: str, w: int, h: int, passes: int, seeds: np.ndarray, canvas: np
     if rc != 0:
         raise RuntimeError(lib().oracle_last_error().decode())
     return canvas, seeds, int(flops.value)
+
+
+def count_flops(spec: str, w: int, h: int, passes: int, seeds: np.ndarray, culled: bool, crop=None,
+                threads: int = 0):
+    """FP32 render of the crop on the counting scalars. Returns (canvas,
+    seeds, fp32 flops, fp64 flops); culled=True counts the culled scan's
+    algorithm (FP64 cull tests, exact tests of the candidates only)."""
+    seeds = np.array(seeds, dtype=np.uint32).reshape(h, w).copy()
+    canvas = np.zeros((h, w, 4), np.float32)
+    x0, y0, cw, ch = crop if crop is not None else (0, 0, w, h)
+    f32, f64 = c_uint64(0), c_uint64(0)
+    rc = lib().oracle_count_flops(spec.encode(), int(culled), w, h, passes, _p(canvas), _p(seeds), x0, y0, cw, ch,
+                                  threads, ctypes.byref(f32), ctypes.byref(f64))
+    if rc != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return canvas, seeds, int(f32.value), int(f64.value)
 
 
 def vec_call(name: str, *arrays, out_len: int = 4):

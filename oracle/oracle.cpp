@@ -19,8 +19,9 @@
 // except in ~2^-28 of cases), so the oracle does not carry one libm's ulp
 // errors. The reference leaves these to its OpenCL implementation's builtins.
 //
-// Parity status per part (which check pins it) is listed in the header of
-// oracle/README.md.
+// Which check pins each part: tests/test_oracle_pins.py (leaf functions
+// against the reference's compiled leaf sources, closed forms and brute
+// force), tests/test_oracle_scenes.py (whole paths against closed forms).
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -818,11 +819,224 @@ template <class R, class F, class G> struct MappedObject : Object<R, F, G> {
     }
 };
 
+// ------------------------------------------------------------ culled-scan flop counting
+// The HIP kernel's nearest-hit scan over a top-level vector of mapped
+// primitives tests each element's world-space bounding sphere against the
+// ray's geodesic in FP64 and runs the exact test only for candidates; each
+// ray then reruns its winner's exact test and maps that one hit back to the
+// world frame (hypertrace_amd/csrc/device/ht/scan.hh). To count the flops of
+// that algorithm, its bounds and cull tests are restated here on a counting
+// double (FP64 flops, g_flops64); the exact tests run on the counting float
+// as everywhere else. Only the flop count uses this path; it selects the same
+// hit as the reference loop below.
+thread_local uint64_t g_flops64 = 0;
+bool g_count_culled = false;
+
+struct CntD {
+    double v;
+    CntD() : v(0) {}
+    CntD(double x) : v(x) {}
+};
+inline CntD operator+(CntD a, CntD b) { ++g_flops64; return CntD(a.v + b.v); }
+inline CntD operator-(CntD a, CntD b) { ++g_flops64; return CntD(a.v - b.v); }
+inline CntD operator*(CntD a, CntD b) { ++g_flops64; return CntD(a.v * b.v); }
+inline CntD operator/(CntD a, CntD b) { ++g_flops64; return CntD(a.v / b.v); }
+inline CntD sqrtd(CntD a) { ++g_flops64; return CntD(std::sqrt(a.v)); }
+
+struct BoundF {
+    float cx = 0, cy = 0, cz = 0, r = -1, ri = -1;
+    bool horo = false;
+};
+
+inline float next_up(float f) { uint32_t u; std::memcpy(&u, &f, 4); ++u; std::memcpy(&f, &u, 4); return f; }
+inline float next_down(float f) { uint32_t u; std::memcpy(&u, &f, 4); --u; std::memcpy(&f, &u, 4); return f; }
+
+// Sphere (cx, cy, cz, r) rounded to FP32: outer radius inflated by the margin
+// and the centre's rounding and rounded up, inner radius shrunk and rounded down.
+inline BoundF make_bound(double cx, double cy, double cz, double r, double margin) {
+    BoundF b;
+    b.cx = (float)cx;
+    b.cy = (float)cy;
+    b.cz = (float)cz;
+    double err = std::fabs((double)b.cx - cx) + std::fabs((double)b.cy - cy) + std::fabs((double)b.cz - cz);
+    double ro = r * (1.0 + margin) + err + 1e-30;
+    b.r = (float)ro;
+    if ((double)b.r < ro) b.r = next_up(b.r);
+    double rin = r * (1.0 - margin) - err;
+    b.ri = rin > 0.0 ? (float)rin : 0.0f;
+    if ((double)b.ri > rin && b.ri > 0.0f) b.ri = next_down(b.ri);
+    return b;
+}
+
+// Bound of a unit hemisphere (kind 4) or of the horosphere z = 1 (kind 5)
+// under the Moebius map [[a, b], [c, d]]: the hemisphere over the image of
+// the unit circle, centre (b conj(d) - a conj(c)) / (|d|^2 - |c|^2), radius
+// 1 / ||d|^2 - |c|^2|; the horoball tangent to the boundary at a / c with
+// diameter 1 / |c|^2. Margin 1e-3 + 1e-4 n2^2 for the squared norm n2.
+template <class R> BoundF mo_bound(const C22<R> &m, int kind) {
+    double ar = tofloat(m.a.x), ai = tofloat(m.a.y), br = tofloat(m.b.x), bi = tofloat(m.b.y);
+    double cr = tofloat(m.c.x), ci = tofloat(m.c.y), dr = tofloat(m.d.x), di = tofloat(m.d.y);
+    double n2 = ar * ar + ai * ai + br * br + bi * bi + cr * cr + ci * ci + dr * dr + di * di;
+    double margin = 1e-3 + 1e-4 * n2 * n2;
+    if (kind == 5) {
+        double c2 = cr * cr + ci * ci;
+        if (!(c2 > 1e-12 * n2)) return BoundF();
+        double rad = 0.5 / c2;
+        BoundF b = make_bound((ar * cr + ai * ci) / c2, (ai * cr - ar * ci) / c2, rad, rad, margin);
+        b.horo = true;
+        return b;
+    }
+    if (kind == 4) {
+        double den = (dr * dr + di * di) - (cr * cr + ci * ci);
+        if (!(std::fabs(den) > 1e-9 * n2)) return BoundF();
+        double nx = (br * dr + bi * di) - (ar * cr + ai * ci);
+        double ny = (bi * dr - br * di) - (ai * cr - ar * ci);
+        return make_bound(nx / den, ny / den, 0.0, 1.0 / std::fabs(den), margin);
+    }
+    return BoundF();
+}
+
+// Straight Euclidean ray p + t d, t >= 0, against a sphere.
+struct CullRayEu {
+    CntD px, py, pz, dx, dy, dz;
+    template <class R> static CullRayEu make(const V3<R> &p, const V3<R> &d) {
+        return {CntD(tofloat(p.x)), CntD(tofloat(p.y)), CntD(tofloat(p.z)),
+                CntD(tofloat(d.x)), CntD(tofloat(d.y)), CntD(tofloat(d.z))};
+    }
+    bool candidate(const BoundF &b) const {
+        if (b.r < 0.f) return true;
+        CntD sx = CntD(b.cx) - px, sy = CntD(b.cy) - py, sz = CntD(b.cz) - pz;
+        CntD r = b.r;
+        CntD t = sx * dx + sy * dy + sz * dz;
+        if (t.v < -r.v) return false;
+        CntD qx = sx - t * dx, qy = sy - t * dy, qz = sz - t * dz;
+        return (qx * qx + qy * qy + qz * qz).v <= (r * r).v;
+    }
+};
+
+// Hyperbolic geodesic from p with tangent d: the semicircle in the vertical
+// plane through p along u = d.xy / |d.xy|, centre p.xy + gam u, radius rg;
+// the part ahead of p. A sphere is culled when the plane misses it, its cut
+// lies wholly behind p, the circles are disjoint, or one lies strictly inside
+// the other; horoballs near the geodesic through their tangent point keep
+// the behind-the-start case (the horosphere test accepts crossings behind
+// the start for near-vertical local rays, shape/hy/horosphere.cc:16-24).
+struct CullRayHy {
+    CntD px, py, pz, dx, dy, dz, ux, uy, gam, rg;
+    bool valid = false;
+    template <class R> static CullRayHy make(const V4<R> &p, const V4<R> &d) {
+        CullRayHy c;
+        CntD x = tofloat(d.x), y = tofloat(d.y), z = tofloat(d.z);
+        CntD pz = tofloat(p.z);
+        CntD dxy = sqrtd(x * x + y * y);
+        CntD dn = sqrtd(x * x + y * y + z * z);
+        c.valid = dxy.v > (CntD(1e-4) * dn).v && pz.v > 0.0;
+        if (!c.valid) dxy = 1.0;
+        c.px = tofloat(p.x);
+        c.py = tofloat(p.y);
+        c.pz = pz;
+        c.dx = x / dn;
+        c.dy = y / dn;
+        c.dz = z / dn;
+        c.ux = x / dxy;
+        c.uy = y / dxy;
+        c.gam = pz * z / dxy;
+        c.rg = pz * dn / dxy;
+        return c;
+    }
+    bool toward_tangent_point(double tx, double ty) const {
+        CntD vx = CntD(tx) - px, vy = CntD(ty) - py;
+        CntD v2 = vx * vx + vy * vy, h = pz;
+        CntD tz = CntD(0.5) * (v2 - h * h);
+        CntD n = h * (dx * vx + dy * vy) + dz * tz;
+        CntD den2 = h * h * v2 + tz * tz;
+        return (den2 - n * n).v < (CntD(1e-2) * den2).v;
+    }
+    bool candidate(const BoundF &b) const {
+        if (b.r < 0.f || !valid) return true;
+        CntD rx = CntD(b.cx) - px, ry = CntD(b.cy) - py;
+        CntD ro = b.r, ri = b.ri;
+        CntD delta = CntD(0.0) - rx * uy + ry * ux;
+        CntD d2p = delta * delta;
+        CntD rp2 = ro * ro - d2p;
+        if (rp2.v < 0.0) return false;
+        CntD sig = rx * ux + ry * uy;
+        if (sig.v < 0.0 && (sig * sig).v > rp2.v && !(b.horo && toward_tangent_point(b.cx, b.cy))) return false;
+        CntD ex = sig - gam;
+        CntD d2 = ex * ex + CntD(b.cz) * CntD(b.cz);
+        CntD rg2 = rg * rg;
+        CntD o = d2 - rg2 - rp2;
+        if (o.v > 0.0 && (o * o).v > (CntD(4.0) * rg2 * rp2).v) return false;
+        if (rg2.v > rp2.v) {
+            CntD in = rg2 + rp2 - d2;
+            if (in.v > 0.0 && (in * in).v > (CntD(4.0) * rg2 * rp2).v) return false;
+        }
+        CntD rpi2 = ri * ri - d2p;
+        if (rpi2.v > rg2.v) {
+            CntD in = rg2 + rpi2 - d2;
+            if (in.v > 0.0 && (in * in).v > (CntD(4.0) * rg2 * rpi2).v) return false;
+        }
+        return true;
+    }
+};
+
+template <class R> inline CullRayEu cull_ray(const Ray<R, GeoEu<R>> &r) { return CullRayEu::make(r.start, r.dir); }
+template <class R> inline CullRayHy cull_ray(const Ray<R, GeoHy<R>> &r) { return CullRayHy::make(r.start, r.dir); }
+
+// World bound of one vector element; false if the element is not a mapped
+// single primitive (then the kernel scans the whole vector plainly).
+template <class R, class F> bool element_bound(const Object<R, F, GeoHy<R>> *o, BoundF &b);
+template <class R, class F> bool element_bound(const Object<R, F, GeoEu<R>> *o, BoundF &b);
+
 // object/vector.inl:24-74
 template <class R, class F, class G> struct VecObject : Object<R, F, G> {
     std::vector<std::unique_ptr<Object<R, F, G>>> items;
     int level = 0;
+    std::vector<BoundF> bounds;
+    bool cullable = false;
+
+    void prepare_cull() {
+        bounds.assign(items.size(), BoundF());
+        cullable = !items.empty() && items.size() <= 4096;
+        for (size_t i = 0; i < items.size() && cullable; ++i) cullable = element_bound(items[i].get(), bounds[i]);
+    }
+
+    // The culled scan's work: cull tests, exact tests of the candidates
+    // without the map back, then the winner's exact test and its map back.
+    R detect_culled(Ctx &ctx, Cache<R, G> &cache, Light<R, F, G> &light) const {
+        const R EPS = Consts<R>::eps();
+        uint32_t orig = ctx.hasher;
+        auto cr = cull_ray(light.ray);
+        R min_dist = R(-1);
+        uint32_t min_index = 0;
+        for (size_t i = 0; i < items.size(); ++i) {
+            if (!cr.candidate(bounds[i])) continue;
+            auto mo = static_cast<const MappedObject<R, F, G> *>(items[i].get());
+            Light<R, F, G> cur = light;
+            Cache<R, G> cc = cache;
+            ctx.hasher = hash_step(orig, (uint32_t)i);
+            cur.ray = ray_map(cur.ray, *mo->inv);
+            R dist = mo->inner->detect(ctx, cc, cur);
+            ctx.hasher = orig;
+            if (dist > -EPS && (min_dist <= -EPS || dist < min_dist)) {
+                min_dist = dist;
+                min_index = (uint32_t)i;
+            }
+        }
+        if (!(min_dist > -EPS)) return R(-1);
+        auto mo = static_cast<const MappedObject<R, F, G> *>(items[min_index].get());
+        Cache<R, G> cc = cache;
+        ctx.hasher = hash_step(orig, min_index);
+        light.ray = ray_map(light.ray, *mo->inv);
+        R dist = mo->inner->detect(ctx, cc, light);
+        light.ray = ray_map(light.ray, *mo->map);
+        cache = cc;
+        cache.idx[level] = min_index;
+        return dist;
+    }
+
     R detect(Ctx &ctx, Cache<R, G> &cache, Light<R, F, G> &light) const override {
+        if (g_count_culled && cullable) return detect_culled(ctx, cache, light);
         const R EPS = Consts<R>::eps();
         uint32_t orig = ctx.hasher;
         R min_dist = R(-1);
@@ -857,6 +1071,53 @@ template <class R, class F, class G> struct VecObject : Object<R, F, G> {
         return items[cache.idx[level]]->interact(ctx, cache, light, e);
     }
 };
+
+template <class R, class F> bool element_bound(const Object<R, F, GeoHy<R>> *o, BoundF &b) {
+    auto mo = dynamic_cast<const MappedObject<R, F, GeoHy<R>> *>(o);
+    if (!mo) return false;
+    auto mm = dynamic_cast<const MoMap<R> *>(mo->map.get());
+    if (!mm) return false;
+    const Object<R, F, GeoHy<R>> *in = mo->inner.get();
+    int kind = 0;
+    if (auto cov = dynamic_cast<const Covered<R, F, GeoHy<R>> *>(in)) {
+        const Shape<R, F, GeoHy<R>> *s = cov->shape.get();
+        if (dynamic_cast<const PlaneHy<R, F> *>(s)) kind = 4;
+        else if (dynamic_cast<const HorosphereHy<R, F> *>(s)) kind = 5;
+    } else if (dynamic_cast<const TiledPlaneObj<R, F> *>(in)) {
+        kind = 4;
+    } else if (dynamic_cast<const TiledHoroObj<R, F> *>(in)) {
+        kind = 5;
+    } else {
+        return false;
+    }
+    b = mo_bound(mm->m, kind);
+    return true;
+}
+
+// Isometric Euclidean maps fix no origin but move it by the shift: a sphere
+// of the primitive's radius (unit sphere 1, ball r, cube sqrt 3) about the
+// shift, inflated by 1e-4 of the shift's L1 norm and by 1e-4 relative.
+template <class R, class F> bool element_bound(const Object<R, F, GeoEu<R>> *o, BoundF &b) {
+    auto mo = dynamic_cast<const MappedObject<R, F, GeoEu<R>> *>(o);
+    if (!mo) return false;
+    auto hm = dynamic_cast<const HomMap<R> *>(mo->map.get());
+    auto cov = dynamic_cast<const Covered<R, F, GeoEu<R>> *>(mo->inner.get());
+    if (!hm || !cov) return false;
+    const Shape<R, F, GeoEu<R>> *s = cov->shape.get();
+    double radius = -1.0;
+    if (auto sp = dynamic_cast<const SphereEu<R, F> *>(s)) radius = sp->unit ? 1.0 : (double)tofloat(sp->radius);
+    else if (dynamic_cast<const CubeEu<R, F> *>(s)) radius = 1.7320508075688772;
+    if (radius < 0.0) {
+        b = BoundF();
+        return true;
+    }
+    double cx = hm->has_shift ? tofloat(hm->shift.x) : 0.0;
+    double cy = hm->has_shift ? tofloat(hm->shift.y) : 0.0;
+    double cz = hm->has_shift ? tofloat(hm->shift.z) : 0.0;
+    double dist = std::fabs(cx) + std::fabs(cy) + std::fabs(cz);
+    b = make_bound(cx, cy, cz, radius + 1e-4 * dist, 1e-4);
+    return true;
+}
 
 // ------------------------------------------------------------ views, backgrounds
 template <class R, class G> struct View {
@@ -1174,14 +1435,16 @@ template <class R, class F, class G> std::unique_ptr<Scene<R, F, G>> parse_scene
     s->view = p.view();
     s->bg = p.background();
     s->object = p.object(0);
+    if (auto v = dynamic_cast<VecObject<R, F, G> *>(s->object.get())) v->prepare_cull();
     return s;
 }
 
 // render.cl:12-26 for each pixel of the crop; `passes` samples per pixel.
 template <class R, class F, class G>
 uint64_t render_impl(const Scene<R, F, G> &scene, uint32_t w, uint32_t h, uint32_t passes, float *canvas,
-                     uint32_t *seeds, uint32_t x0, uint32_t y0, uint32_t cw, uint32_t ch, int threads) {
-    std::atomic<uint64_t> total{0};
+                     uint32_t *seeds, uint32_t x0, uint32_t y0, uint32_t cw, uint32_t ch, int threads,
+                     uint64_t *flops64 = nullptr) {
+    std::atomic<uint64_t> total{0}, total64{0};
     long n = (long)cw * (long)ch;
 #ifdef _OPENMP
     if (threads <= 0) threads = omp_get_max_threads();
@@ -1189,6 +1452,7 @@ uint64_t render_impl(const Scene<R, F, G> &scene, uint32_t w, uint32_t h, uint32
 #endif
     for (long k = 0; k < n; ++k) {
         g_flops = 0;
+        g_flops64 = 0;
         uint32_t x = x0 + (uint32_t)(k % cw), y = y0 + (uint32_t)(k / cw);
         uint32_t idx = x + w * y;
         V2<R> pos = V2<R>{R(double(2 * x + 1)) - R(double(w)), R(double(h)) - R(double(2 * y + 1))} / R(double(h));
@@ -1208,23 +1472,25 @@ uint64_t render_impl(const Scene<R, F, G> &scene, uint32_t w, uint32_t h, uint32
         }
         seeds[idx] = ctx.rng.state;
         total += g_flops;
+        total64 += g_flops64;
     }
+    if (flops64) *flops64 = total64.load();
     return total.load();
 }
 
 template <class R, class F>
 uint64_t render_any(const char *spec, uint32_t w, uint32_t h, uint32_t passes, float *canvas, uint32_t *seeds,
-                    uint32_t x0, uint32_t y0, uint32_t cw, uint32_t ch, int threads) {
+                    uint32_t x0, uint32_t y0, uint32_t cw, uint32_t ch, int threads, uint64_t *flops64 = nullptr) {
     Tokens tk(spec);
     tk.expect("scene");
     std::string geo = tk.next();
     if (geo == "eu") {
         auto s = parse_scene<R, F, GeoEu<R>>(tk);
-        return render_impl(*s, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads);
+        return render_impl(*s, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads, flops64);
     }
     if (geo == "hy") {
         auto s = parse_scene<R, F, GeoHy<R>>(tk);
-        return render_impl(*s, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads);
+        return render_impl(*s, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads, flops64);
     }
     throw std::runtime_error("unknown geometry " + geo);
 }
@@ -1261,6 +1527,29 @@ int oracle_render(const char *spec, int precision, uint32_t w, uint32_t h, uint3
         if (flops) *flops = f;
         return 0;
     } catch (const std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// Flop count of the FP32 render over a crop: *flops32 = FP32 flops, *flops64
+// = FP64 flops. culled = 0: the reference algorithm (every element's exact
+// test, each mapped back); culled = 1: the culled scan's algorithm (FP64
+// cull tests, exact tests of candidates, one map back per hit).
+int oracle_count_flops(const char *spec, int culled, uint32_t w, uint32_t h, uint32_t passes, float *canvas,
+                       uint32_t *seeds, uint32_t x0, uint32_t y0, uint32_t cw, uint32_t ch, int threads,
+                       uint64_t *flops32, uint64_t *flops64) {
+    try {
+        if (x0 + cw > w || y0 + ch > h) throw std::runtime_error("crop outside the image");
+        g_count_culled = culled != 0;
+        uint64_t f64 = 0;
+        uint64_t f32 = render_any<Cnt, Cnt>(spec, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads, &f64);
+        g_count_culled = false;
+        if (flops32) *flops32 = f32;
+        if (flops64) *flops64 = f64;
+        return 0;
+    } catch (const std::exception &e) {
+        g_count_culled = false;
         g_err = e.what();
         return -1;
     }

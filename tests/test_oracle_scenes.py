@@ -153,3 +153,19 @@ def test_hyperbolic_isometry_invariance():
     ok = np.all(np.abs(a - b) <= 1e-6 * np.abs(a) + 1e-7, axis=-1)
     assert ok.mean() >= 0.99
     assert (a[..., 0] < 0.999).mean() > 0.2  # objects are in view
+
+
+def test_culled_flop_count_keeps_the_render():
+    """The culled-scan flop count (the roofline's numerator) runs the FP64
+    cull before the exact tests; its render must equal the plain count's, hit
+    for hit, and it must count fewer FP32 flops plus some FP64 ones, in both
+    geometries."""
+    from hypertrace_amd import scenes
+
+    for wl in (scenes.hyperbolic_cells(64, 48, 1, cells=16), scenes.euclidean_spheres(64, 48, 1)):
+        spec = wl.scene.spec_text(X32_F32)
+        seeds = seed_fill(64 * 48, wl.seed_mode, wl.seed)
+        c0, s0, p32, p64 = oracle.count_flops(spec, 64, 48, 2, seeds, False)
+        c1, s1, c32, c64 = oracle.count_flops(spec, 64, 48, 2, seeds, True)
+        assert np.array_equal(c0, c1) and np.array_equal(s0, s1), wl.name
+        assert p64 == 0 and c64 > 0 and c32 < p32, (wl.name, p32, c32, c64)
