@@ -208,6 +208,14 @@ def main():
 
     with Context(0, X32_F32) as ctx:
         buf = EntityBuffer(ctx, wl.scene)
+        # The 192 cell maps are composed on the GPU in FP64 (det = 1
+        # reprojection after every product) straight into the scene buffer;
+        # they must equal the host composition bit for bit.
+        from hypertrace_amd.maps import compose_on_device
+        compose_on_device(ctx, buf, wl.scene, wl.word_maps)
+        maps_ok = buf.read().tobytes() == wl.scene.to_bytes(X32_F32)
+        if not maps_ok:
+            raise RuntimeError("GPU-composed scene maps differ from the host composition")
 
         def time_chosen(render, steps, warmup):
             canvas = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
@@ -291,7 +299,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "hyperbolic_cells", "width": W, "height": H, "spp": SPP, "hops": 8,
-                   "primitives": 192, "kernel": "render_passes"},
+                   "primitives": 192, "kernel": "render_passes",
+                   "maps": "192 Moebius words composed on the GPU in FP64 with det=1 reprojection, bit-identical "
+                           "to the host composition"},
         "roofline": roofline,
     }
     out.update(extra)

@@ -9,6 +9,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <atomic>
 #include <cerrno>
 #include <cstdint>
 #include <cstdio>
@@ -95,11 +96,30 @@ void write_file_atomic(const std::string &dir, const std::string &path, const st
 
 }  // namespace
 
+// A context is reference counted like an OpenCL cl_context: its owner holds
+// one reference and every buffer and program retains one, so destroying the
+// context while buffers or programs are alive (e.g. freed later by a garbage
+// collector) keeps the stream valid until the last of them is released.
 struct ht_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    std::atomic<int> refs{1};
 };
+
+namespace {
+void ctx_retain(ht_ctx *ctx) { ctx->refs.fetch_add(1); }
+void ctx_release(ht_ctx *ctx) {
+    if (ctx->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->t0);
+    (void)hipEventDestroy(ctx->t1);
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipGetLastError();  // teardown errors are not reported to later calls
+    delete ctx;
+}
+}  // namespace
 
 struct ht_prog {
     ht_ctx *ctx = nullptr;
@@ -163,6 +183,86 @@ __global__ void __launch_bounds__(256) k_post(uint32_t n, uint32_t passes, float
     float4 e = make_float4(c.x / p, c.y / p, c.z / p, c.w / p);
     float4 v = make_float4(gamma1(e.x, g), gamma1(e.y, g), gamma1(e.z, g), gamma1(e.w, g));
     rgba8[i] = pack4(v);
+}
+
+// ------------------------------------------------------------------ Moebius words
+
+// FP64 composition of Moebius maps with det = 1 reprojection after every
+// product. Every operation rounds on its own (no contraction) and the square
+// roots are IEEE, so the result is bit-identical to the same operations in
+// any IEEE double arithmetic (the host mirror in hypertrace_amd/maps.py).
+struct Cd {
+    double x, y;
+};
+__device__ Cd cd_mul(Cd a, Cd b) {
+#pragma clang fp contract(off)
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ Cd cd_add(Cd a, Cd b) {
+#pragma clang fp contract(off)
+    return {a.x + b.x, a.y + b.y};
+}
+__device__ Cd cd_sub(Cd a, Cd b) {
+#pragma clang fp contract(off)
+    return {a.x - b.x, a.y - b.y};
+}
+// Principal square root, the cancellation-free branch for each half-plane.
+__device__ Cd cd_sqrt(Cd z) {
+#pragma clang fp contract(off)
+    double r = __builtin_sqrt(z.x * z.x + z.y * z.y);
+    if (z.x >= 0.0) {
+        double re = __builtin_sqrt((r + z.x) * 0.5);
+        return {re, z.y / (re + re)};
+    }
+    double im = __builtin_sqrt((r - z.x) * 0.5);
+    if (z.y < 0.0) im = -im;
+    return {z.y / (im + im), im};
+}
+struct C22d {
+    Cd a, b, c, d;
+};
+__device__ C22d c22d_mul(const C22d &p, const C22d &q) {
+    return {cd_add(cd_mul(p.a, q.a), cd_mul(p.b, q.c)), cd_add(cd_mul(p.a, q.b), cd_mul(p.b, q.d)),
+            cd_add(cd_mul(p.c, q.a), cd_mul(p.d, q.c)), cd_add(cd_mul(p.c, q.b), cd_mul(p.d, q.d))};
+}
+// m / sqrt(det m): det = 1 reprojection (c22_normalize, algebra/matrix_complex.cc:50-52,
+// with an algebraic square root in place of the polar form).
+__device__ C22d c22d_renorm(const C22d &m) {
+#pragma clang fp contract(off)
+    Cd s = cd_sqrt(cd_sub(cd_mul(m.a, m.d), cd_mul(m.b, m.c)));
+    double s2 = s.x * s.x + s.y * s.y;
+    Cd inv = {s.x / s2, -s.y / s2};
+    return {cd_mul(m.a, inv), cd_mul(m.b, inv), cd_mul(m.c, inv), cd_mul(m.d, inv)};
+}
+__device__ C22d c22d_load(const double *v) { return {{v[0], v[1]}, {v[2], v[3]}, {v[4], v[5]}, {v[6], v[7]}}; }
+
+// Element i: M = N(N(...N(N(I G[w0]) G[w1])...) P_i) over its word w (N =
+// reprojection), stored as 8 reals (a, b, c, d as (re, im) pairs) at
+// scene + offsets[i], in FP32 (round to nearest) or FP64.
+__global__ void __launch_bounds__(64) k_mo_compose(uint8_t *__restrict__ scene, uint32_t n, int real_size,
+                                                   const uint32_t *__restrict__ offsets,
+                                                   const uint8_t *__restrict__ letters,
+                                                   const uint8_t *__restrict__ lengths,
+                                                   const double *__restrict__ gens,
+                                                   const double *__restrict__ posts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    C22d m = {{1.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {1.0, 0.0}};
+    uint32_t len = lengths[i];
+    for (uint32_t k = 0; k < len; ++k) {
+        C22d g = c22d_load(gens + 8 * (size_t)letters[(size_t)i * HT_MO_WORD_MAX + k]);
+        m = c22d_renorm(c22d_mul(m, g));
+    }
+    m = c22d_renorm(c22d_mul(m, c22d_load(posts + 8 * (size_t)i)));
+    double v[8] = {m.a.x, m.a.y, m.b.x, m.b.y, m.c.x, m.c.y, m.d.x, m.d.y};
+    uint8_t *dst = scene + offsets[i];
+    if (real_size == 4) {
+        float *f = reinterpret_cast<float *>(dst);
+        for (int k = 0; k < 8; ++k) f[k] = (float)v[k];
+    } else {
+        double *d = reinterpret_cast<double *>(dst);
+        for (int k = 0; k < 8; ++k) d[k] = v[k];
+    }
 }
 
 // ------------------------------------------------------------------ seeds
@@ -257,7 +357,7 @@ extern "C" int ht_ctx_create(int device_index, ht_ctx **out) {
         return fail(HT_ERR_BACKEND, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) {
-        hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(HT_ERR_BACKEND, "hipEventCreate failed");
     }
@@ -267,12 +367,7 @@ extern "C" int ht_ctx_create(int device_index, ht_ctx **out) {
 
 extern "C" void ht_ctx_destroy(ht_ctx *ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    hipEventDestroy(ctx->t0);
-    hipEventDestroy(ctx->t1);
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctx_release(ctx);
 }
 
 extern "C" int ht_ctx_sync(ht_ctx *ctx) {
@@ -383,10 +478,11 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
     }
     if (hipModuleGetFunction(&p->render, p->module, "render") != hipSuccess ||
         hipModuleGetFunction(&p->render_passes, p->module, "render_passes") != hipSuccess) {
-        hipModuleUnload(p->module);
+        (void)hipModuleUnload(p->module);
         delete p;
         return fail(HT_ERR_BACKEND, "program lacks the render/render_passes kernels");
     }
+    ctx_retain(ctx);
     *out = p;
     return HT_OK;
 }
@@ -408,9 +504,12 @@ extern "C" int ht_prog_precompile(const char *root_name, const char *root_src, i
 
 extern "C" void ht_prog_destroy(ht_prog *prog) {
     if (!prog) return;
-    hipSetDevice(prog->ctx->device);
-    hipModuleUnload(prog->module);
+    (void)hipSetDevice(prog->ctx->device);
+    (void)hipModuleUnload(prog->module);
+    (void)hipGetLastError();
+    ht_ctx *ctx = prog->ctx;
     delete prog;
+    ctx_release(ctx);
 }
 
 extern "C" int ht_buf_alloc(ht_ctx *ctx, size_t bytes, ht_buf **out) {
@@ -426,16 +525,20 @@ extern "C" int ht_buf_alloc(ht_ctx *ctx, size_t bytes, ht_buf **out) {
             return fail(HT_ERR_BACKEND, std::string("hipMalloc: ") + hipGetErrorString(e));
         }
     }
+    ctx_retain(ctx);
     *out = b;
     return HT_OK;
 }
 
 extern "C" int ht_buf_free(ht_buf *buf) {
     if (!buf) return HT_OK;
-    HIP_TRY(hipSetDevice(buf->ctx->device));
-    HIP_TRY(hipStreamSynchronize(buf->ctx->stream));
-    if (buf->ptr) HIP_TRY(hipFree(buf->ptr));
+    ht_ctx *ctx = buf->ctx;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && buf->ptr) e = hipFree(buf->ptr);
     delete buf;
+    ctx_release(ctx);
+    if (e != hipSuccess) return fail(HT_ERR_BACKEND, std::string("ht_buf_free: ") + hipGetErrorString(e));
     return HT_OK;
 }
 
@@ -524,6 +627,7 @@ extern "C" int ht_extract(ht_ctx *ctx, uint32_t w, uint32_t h, uint32_t passes, 
     uint32_t n = w * h;
     if (!n) return HT_OK;
     HIP_TRY(hipSetDevice(ctx->device));
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_extract, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, 0.0f, passes,
                        (const float4 *)canvas->ptr, (float4 *)image->ptr);
     HIP_TRY(hipGetLastError());
@@ -538,6 +642,7 @@ extern "C" int ht_gamma(ht_ctx *ctx, uint32_t w, uint32_t h, float gamma, ht_buf
     uint32_t n = w * h;
     if (!n) return HT_OK;
     HIP_TRY(hipSetDevice(ctx->device));
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_gamma, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, gamma,
                        (const float4 *)in->ptr, (float4 *)out->ptr);
     HIP_TRY(hipGetLastError());
@@ -552,6 +657,7 @@ extern "C" int ht_pack(ht_ctx *ctx, uint32_t w, uint32_t h, ht_buf *image, ht_bu
     uint32_t n = w * h;
     if (!n) return HT_OK;
     HIP_TRY(hipSetDevice(ctx->device));
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, (const float4 *)image->ptr,
                        (uint32_t *)rgba8->ptr);
     HIP_TRY(hipGetLastError());
@@ -567,9 +673,56 @@ extern "C" int ht_post(ht_ctx *ctx, uint32_t w, uint32_t h, uint32_t passes, flo
     uint32_t n = w * h;
     if (!n) return HT_OK;
     HIP_TRY(hipSetDevice(ctx->device));
+    (void)hipGetLastError();
     hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, passes, gamma,
                        (const float4 *)canvas->ptr, (uint32_t *)rgba8->ptr);
     HIP_TRY(hipGetLastError());
+    return HT_OK;
+}
+
+extern "C" int ht_mo_compose(ht_ctx *ctx, ht_buf *scene, int real_size, uint32_t n, const uint32_t *offsets,
+                             const uint8_t *letters, const uint8_t *lengths, const double *gens, uint32_t n_gens,
+                             const double *posts) {
+    if (!ctx || !scene) return fail(HT_ERR_OTHER, "ht_mo_compose: null ctx or scene");
+    if (real_size != 4 && real_size != 8) return fail(HT_ERR_OTHER, "ht_mo_compose: real_size must be 4 or 8");
+    if (!n) return HT_OK;
+    if (!offsets || !letters || !lengths || !gens || !posts || !n_gens)
+        return fail(HT_ERR_OTHER, "ht_mo_compose: null array");
+    for (uint32_t i = 0; i < n; ++i) {
+        if ((size_t)offsets[i] + 8 * (size_t)real_size > scene->size || offsets[i] % (8 * real_size))
+            return fail(HT_ERR_OTHER, "ht_mo_compose: map offset " + std::to_string(offsets[i]) +
+                                          " outside the scene buffer or misaligned");
+        if (lengths[i] > HT_MO_WORD_MAX) return fail(HT_ERR_OTHER, "ht_mo_compose: word longer than HT_MO_WORD_MAX");
+        for (uint32_t k = 0; k < lengths[i]; ++k)
+            if (letters[(size_t)i * HT_MO_WORD_MAX + k] >= n_gens)
+                return fail(HT_ERR_OTHER, "ht_mo_compose: letter outside the generator set");
+    }
+    // One staging allocation for the inputs: offsets | lengths | letters | gens | posts.
+    size_t o_off = 0, o_len = o_off + 4 * (size_t)n, o_let = o_len + n;
+    size_t o_gen = (o_let + (size_t)n * HT_MO_WORD_MAX + 7) & ~(size_t)7, o_post = o_gen + 64 * (size_t)n_gens;
+    size_t total = o_post + 64 * (size_t)n;
+    std::vector<uint8_t> host(total);
+    std::memcpy(host.data() + o_off, offsets, 4 * (size_t)n);
+    std::memcpy(host.data() + o_len, lengths, n);
+    std::memcpy(host.data() + o_let, letters, (size_t)n * HT_MO_WORD_MAX);
+    std::memcpy(host.data() + o_gen, gens, 64 * (size_t)n_gens);
+    std::memcpy(host.data() + o_post, posts, 64 * (size_t)n);
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint8_t *dev = nullptr;
+    HIP_TRY(hipMalloc(&dev, total));
+    hipError_t e = hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        (void)hipGetLastError();
+        hipLaunchKernelGGL(k_mo_compose, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, (uint8_t *)scene->ptr, n,
+                           real_size, (const uint32_t *)(dev + o_off), dev + o_let, dev + o_len,
+                           (const double *)(dev + o_gen), (const double *)(dev + o_post));
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    hipError_t e3 = hipFree(dev);
+    if (e != hipSuccess) return fail(HT_ERR_BACKEND, std::string("ht_mo_compose: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(HT_ERR_BACKEND, std::string("ht_mo_compose: ") + hipGetErrorString(e2));
+    if (e3 != hipSuccess) return fail(HT_ERR_BACKEND, std::string("ht_mo_compose: ") + hipGetErrorString(e3));
     return HT_OK;
 }
 

@@ -38,6 +38,8 @@ SIGNATURES = {
     "ht_gamma": (c_int, [c_void_p, c_uint32, c_uint32, c_float, c_void_p, c_void_p]),
     "ht_pack": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p]),
     "ht_post": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, c_float, c_void_p, c_void_p]),
+    "ht_mo_compose": (c_int, [c_void_p, c_void_p, c_int, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32,
+                              c_void_p]),
     "ht_seed_init": (c_int, [c_void_p, c_uint32, c_uint32, c_int, c_uint64]),
     "ht_seed_fill": (c_int, [POINTER(c_uint32), c_size_t, c_int, c_uint64]),
     "ht_seed_hash": (c_uint32, [c_uint64, c_uint32]),

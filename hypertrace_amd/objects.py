@@ -147,8 +147,19 @@ def _struct_size(cfg, fields, dynamic: bool) -> int:
 def _struct_store(cfg, w, fields, align) -> None:
     for f in fields:
         w.align(f.align(cfg))
+        if w.marks is not None:
+            w.marks.setdefault(id(f), w.position)
         f.store(cfg, w)
     w.align(align)
+
+
+def field_offsets(root: "Entity", cfg: Config, targets) -> List[int]:
+    """Byte offsets, in root's serialized form, of the given entity instances
+    (each stored as a field of some struct, e.g. the map of a Mapped)."""
+    w = Writer(cfg)
+    w.marks = {}
+    root.store(cfg, w)
+    return [w.marks[id(t)] for t in targets]
 
 
 # --------------------------------------------------------------------- entity base

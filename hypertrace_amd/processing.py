@@ -202,6 +202,12 @@ class Program:
             native.lib().ht_prog_destroy(self.handle)
             self.handle = None
 
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 # Program FP modes (see include/hypertrace_hip.h, ht_prog_build_ex).
 STRICT_FP = "-ffp-contract=off -DHT_STRICT_FP=1"  # the default

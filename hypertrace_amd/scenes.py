@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import List, Tuple
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -64,6 +64,8 @@ class Workload:
     spp: int
     seed_mode: int
     seed: int
+    # Scene maps given as words over generators (GPU-composable), if any.
+    word_maps: Optional["WordMaps"] = None
 
 
 def _rng(k: int) -> np.random.Generator:
@@ -118,14 +120,16 @@ CELL_GENERATORS = (
 def hyperbolic_cells(width: int = 640, height: int = 480, spp: int = 64, cells: int = 64,
                      hops: int = 8) -> Workload:
     """The hyperbolic cell scene; every primitive's map is the FP64 product
-    word * base, normalised to det 1."""
+    word * base with det = 1 reprojection after each factor
+    (maps.compose_word, bit-identical to the GPU composition)."""
+    from .maps import WordMaps, compose_word
+
     rng = _rng(2)
     choice = [Plane(), Horosphere()]
     objs = []
+    wm = WordMaps(list(CELL_GENERATORS), [], [], [])
     for _ in range(cells):
-        word = Moebius.identity()
-        for _ in range(int(rng.integers(1, 7))):
-            word = word.chain(CELL_GENERATORS[int(rng.integers(0, 4))])
+        letters = [int(rng.integers(0, 4)) for _ in range(int(rng.integers(1, 7)))]
         # Cell layout below the camera: a horosphere of diameter dh at the
         # cell centre and two half-sphere planes beside it.
         dh = rng.uniform(0.15, 0.35)
@@ -135,14 +139,17 @@ def hyperbolic_cells(width: int = 640, height: int = 480, spp: int = 64, cells: 
             (0, _translate(complex(*rng.uniform(-0.5, 0.5, 2))).chain(_scale(rng.uniform(0.1, 0.3)))),
         ]
         for kind, local in locals_:
-            m = word.chain(local).normalized()
+            m = compose_word(CELL_GENERATORS, letters, local)
+            wm.words.append(letters)
+            wm.posts.append(local)
+            wm.targets.append(m)
             albedo = rng.uniform(0.3, 0.9, 3)
             material = Mixture([(Colored(Lambertian(), albedo), 0.7), (Specular(), 0.3)])
             objs.append(Mapped(Covered(ShapeChoice(choice, kind), material), m))
     view = Mapped(PointView(1.0), hy_zshift(math.log(2.0)))
     bg = ConstBg((1.0, 1.0, 1.0))
     scene = SceneImpl(HY, view, ObjectVec(objs), bg, hops)
-    return Workload("hyperbolic_cells", scene, width, height, spp, 1, MASTER_SEED ^ 2)
+    return Workload("hyperbolic_cells", scene, width, height, spp, 1, MASTER_SEED ^ 2, wm)
 
 
 def _unpack_color(rgb: int):

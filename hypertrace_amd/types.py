@@ -56,6 +56,7 @@ class Writer:
     def __init__(self, cfg: Config):
         self.cfg = cfg
         self.buf = bytearray()
+        self.marks = None  # dict id(entity) -> byte offset, when recording
 
     @property
     def position(self) -> int:

@@ -105,6 +105,22 @@ int ht_pack(ht_ctx *ctx, uint32_t w, uint32_t h, ht_buf *image, ht_buf *rgba8);
 int ht_post(ht_ctx *ctx, uint32_t w, uint32_t h, uint32_t passes, float gamma, ht_buf *canvas,
             ht_buf *rgba8);
 
+/* Scene maps built on the GPU in FP64. Replaces the host-side composition of
+ * hyperbolic isometries in f64 and their narrowing to the device `real`
+ * (the vecmat/ccgeom products the examples chain, main/examples/hy.rs:37-183,
+ * shipped through types/src/primitive/scalar.rs:198-207). For element i:
+ *   M = N(...N(N(I * G[w0]) * G[w1]) ... * P_i),  N(m) = m / sqrt(det m)
+ * (the det = 1 reprojection of c22_normalize, algebra/matrix_complex.cc:50-52)
+ * over its word w = letters[i*HT_MO_WORD_MAX .. + lengths[i]) of generator
+ * indices, all in FP64 with each operation rounded on its own, written as 8
+ * reals (a, b, c, d as (re, im)) at byte offset offsets[i] of `scene`, in FP32
+ * (real_size 4, round to nearest) or FP64 (real_size 8).
+ * gens: n_gens x 8 doubles; posts: n x 8 doubles. Synchronous. */
+#define HT_MO_WORD_MAX 16
+int ht_mo_compose(ht_ctx *ctx, ht_buf *scene, int real_size, uint32_t n, const uint32_t *offsets,
+                  const uint8_t *letters, const uint8_t *lengths, const double *gens, uint32_t n_gens,
+                  const double *posts);
+
 /* Per-pixel seed buffer. Replaces Canvas::new's seed fill
  * (processing/src/canvas.rs:73-84).
  * mode 0: rand 0.8 SmallRng::seed_from_u64(seed).fill(&mut [u32]) on 64-bit
