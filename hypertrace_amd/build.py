@@ -34,7 +34,8 @@ def run(cmd, **kw):
 def build_library() -> str:
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     tmp = LIB + ".tmp"
-    run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(PKG, "csrc", "runtime.hip"), "-L/opt/rocm/lib", "-lhiprtc",
+    run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(PKG, "csrc", "runtime.hip"), os.path.join(PKG, "csrc", "cache.hip"),
+         "-L/opt/rocm/lib", "-lhiprtc",
          "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
     os.replace(tmp, LIB)
     return LIB

@@ -14,7 +14,7 @@ import glob
 import os
 from typing import Dict, List, Tuple
 
-from .objects import Entity, Mapped, SceneImpl, _ChoiceBase, _Component, _Tiled, _VecBase
+from .objects import Covered, Entity, Mapped, ObjectVec, SceneImpl, _ChoiceBase, _Component, _Tiled, _VecBase
 from .types import Config
 
 DEVICE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "device")
@@ -78,13 +78,24 @@ def bounds_cap(scene: SceneImpl) -> int:
     return min(cap, 4096)
 
 
+def cache_supported(scene: SceneImpl) -> bool:
+    """The radiance-cache kernel needs a top-level vector of mapped covered
+    objects (it reads the hit's normal and material through them)."""
+    obj = scene.object
+    return isinstance(obj, ObjectVec) and len(obj.items) > 0 and all(
+        isinstance(e, Mapped) and isinstance(e.inner, Covered) for e in obj.items)
+
+
 def root_source(scene: SceneImpl, cfg: Config) -> str:
+    cached = cache_supported(scene)
     lines = [
         "// Generated render program for one scene type.",
         f"#define HT_BOUNDS_CAP {bounds_cap(scene)}",
-        "#include <ht/render.hh>",
+        "#include <ht/cache.hh>" if cached else "#include <ht/render.hh>",
         f"typedef {scene.cpp()} Scene;",
     ]
     lines += layout_asserts(scene, cfg)
     lines.append("HT_DEFINE_RENDER_KERNELS(Scene)")
+    if cached:
+        lines.append("HT_DEFINE_CACHED_KERNEL(Scene)")
     return "\n".join(lines) + "\n"

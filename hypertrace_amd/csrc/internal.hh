@@ -1,0 +1,66 @@
+// Internal types of the C-ABI runtime shared by its source files
+// (runtime.hip: contexts, buffers, programs, post passes; cache.hip: the
+// radiance cache). Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <string>
+
+#include "../../include/hypertrace_hip.h"
+
+namespace ht_internal {
+
+// Records the message for ht_last_error and returns the code.
+int fail(int code, const std::string &msg);
+
+}  // namespace ht_internal
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            return ht_internal::fail(HT_ERR_BACKEND, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+        }                                                                                               \
+    } while (0)
+
+// A context is reference counted like an OpenCL cl_context: its owner holds
+// one reference and every buffer, program and cache retains one, so
+// destroying the context while they are alive (e.g. freed later by a garbage
+// collector) keeps the stream valid until the last of them is released.
+struct ht_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    std::atomic<int> refs{1};
+};
+
+struct ht_prog {
+    ht_ctx *ctx = nullptr;
+    hipModule_t module = nullptr;
+    hipFunction_t render = nullptr;
+    hipFunction_t render_passes = nullptr;
+    hipFunction_t render_cached = nullptr;  // null unless the scene type supports the radiance cache
+};
+
+struct ht_buf {
+    ht_ctx *ctx = nullptr;
+    void *ptr = nullptr;
+    size_t size = 0;
+};
+
+namespace ht_internal {
+
+inline void ctx_retain(ht_ctx *ctx) { ctx->refs.fetch_add(1); }
+inline void ctx_release(ht_ctx *ctx) {
+    if (ctx->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->t0);
+    (void)hipEventDestroy(ctx->t1);
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipGetLastError();  // teardown errors are not reported to later calls
+    delete ctx;
+}
+
+}  // namespace ht_internal

@@ -20,25 +20,23 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hypertrace_hip.h"
+#include "internal.hh"
 
-namespace {
+using ht_internal::ctx_release;
+using ht_internal::ctx_retain;
+using ht_internal::fail;
 
+namespace ht_internal {
 thread_local std::string g_err = "";
-thread_local int g_cache_hit = 0;
-
 int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+}  // namespace ht_internal
 
-#define HIP_TRY(expr)                                                                                   \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess) {                                                                         \
-            return fail(HT_ERR_BACKEND, std::string(#expr) + ": " + hipGetErrorString(e_));             \
-        }                                                                                               \
-    } while (0)
+namespace {
+
+thread_local int g_cache_hit = 0;
 
 const char *kArch = "gfx950";
 const char *kCacheTag = "ht-codeobj-v1";
@@ -95,44 +93,6 @@ void write_file_atomic(const std::string &dir, const std::string &path, const st
 }
 
 }  // namespace
-
-// A context is reference counted like an OpenCL cl_context: its owner holds
-// one reference and every buffer and program retains one, so destroying the
-// context while buffers or programs are alive (e.g. freed later by a garbage
-// collector) keeps the stream valid until the last of them is released.
-struct ht_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    std::atomic<int> refs{1};
-};
-
-namespace {
-void ctx_retain(ht_ctx *ctx) { ctx->refs.fetch_add(1); }
-void ctx_release(ht_ctx *ctx) {
-    if (ctx->refs.fetch_sub(1) != 1) return;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipEventDestroy(ctx->t0);
-    (void)hipEventDestroy(ctx->t1);
-    (void)hipStreamDestroy(ctx->stream);
-    (void)hipGetLastError();  // teardown errors are not reported to later calls
-    delete ctx;
-}
-}  // namespace
-
-struct ht_prog {
-    ht_ctx *ctx = nullptr;
-    hipModule_t module = nullptr;
-    hipFunction_t render = nullptr;
-    hipFunction_t render_passes = nullptr;
-};
-
-struct ht_buf {
-    ht_ctx *ctx = nullptr;
-    void *ptr = nullptr;
-    size_t size = 0;
-};
 
 // ------------------------------------------------------------------ post kernels
 
@@ -339,7 +299,7 @@ extern "C" int ht_seed_fill(uint32_t *dst, size_t n, int mode, uint64_t seed) {
 
 // ------------------------------------------------------------------ API
 
-extern "C" const char *ht_last_error(void) { return g_err.c_str(); }
+extern "C" const char *ht_last_error(void) { return ht_internal::g_err.c_str(); }
 
 extern "C" int ht_ctx_create(int device_index, ht_ctx **out) {
     if (!out) return fail(HT_ERR_OTHER, "ht_ctx_create: null out");
@@ -481,6 +441,11 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
         (void)hipModuleUnload(p->module);
         delete p;
         return fail(HT_ERR_BACKEND, "program lacks the render/render_passes kernels");
+    }
+    // Present only for scene types the radiance cache supports.
+    if (hipModuleGetFunction(&p->render_cached, p->module, "render_cached") != hipSuccess) {
+        p->render_cached = nullptr;
+        (void)hipGetLastError();
     }
     ctx_retain(ctx);
     *out = p;

@@ -40,6 +40,17 @@ SIGNATURES = {
     "ht_post": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, c_float, c_void_p, c_void_p]),
     "ht_mo_compose": (c_int, [c_void_p, c_void_p, c_int, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p, c_uint32,
                               c_void_p]),
+    "ht_cache_create": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, POINTER(c_void_p)]),
+    "ht_cache_destroy": (None, [c_void_p]),
+    "ht_cache_set_params": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ht_cache_get_params": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ht_cache_get_grads": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ht_cache_counts": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
+    "ht_mlp_forward": (c_int, [c_void_p, c_void_p, c_uint32, c_void_p]),
+    "ht_mlp_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_uint32, c_float, c_int, POINTER(c_float)]),
+    "ht_render_cached": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32,
+                                 c_uint32, c_uint32]),
+    "ht_cache_train": (c_int, [c_void_p, c_uint32, c_uint32, c_float]),
     "ht_seed_init": (c_int, [c_void_p, c_uint32, c_uint32, c_int, c_uint64]),
     "ht_seed_fill": (c_int, [POINTER(c_uint32), c_size_t, c_int, c_uint64]),
     "ht_seed_hash": (c_uint32, [c_uint64, c_uint32]),

@@ -121,6 +121,56 @@ int ht_mo_compose(ht_ctx *ctx, ht_buf *scene, int real_size, uint32_t n, const u
                   const uint8_t *letters, const uint8_t *lengths, const double *gens, uint32_t n_gens,
                   const double *posts);
 
+/* ---- Learned radiance cache. No reference counterpart (the reference has no
+ * learned part); these extend the render pass with a cached method whose
+ * error is judged against the FP64 program.
+ *
+ * MLP: 12 features (position, direction, normal, albedo), each encoded as
+ * sin / cos of 2^b pi x for b < 6 (144 inputs), 5 hidden layers of 64 with
+ * ReLU, 3 outputs (RGB radiance); BF16 weights and activations on MFMA with
+ * FP32 accumulation; FP32 master parameters laid out as
+ *   W0[64][144] b0[64] (W_l[64][64] b_l[64], l = 1..4) W5[3][64] b5[3].
+ * Loss: relative L2, mean over samples x channels of (p - y)^2 / (p^2 + 0.01).
+ * Adam: beta1 0.9, beta2 0.99, eps 1e-8, bias-corrected. */
+#define HT_MLP_PARAMS 26115
+#define HT_MLP_FEATURES 12
+typedef struct ht_cache ht_cache;
+
+/* Query queue of `query_capacity` entries (>= pixels per pass), training
+ * queue of `train_capacity` records, training batches up to `max_batch`
+ * samples (a multiple of 64). Parameters start at zero. */
+int ht_cache_create(ht_ctx *ctx, uint32_t query_capacity, uint32_t train_capacity, uint32_t max_batch,
+                    ht_cache **out);
+void ht_cache_destroy(ht_cache *cache);
+/* Master parameters (HT_MLP_PARAMS floats); setting them resets Adam. */
+int ht_cache_set_params(ht_cache *cache, const float *params, size_t n);
+int ht_cache_get_params(ht_cache *cache, float *params, size_t n);
+/* Gradient of the last training batch (before any Adam update). */
+int ht_cache_get_grads(ht_cache *cache, float *grads, size_t n);
+/* Entries appended to the query and training queues by the last pass
+ * (training records beyond the capacity are dropped, the count goes on). */
+int ht_cache_counts(ht_cache *cache, uint32_t *queries, uint32_t *records);
+
+/* Network on explicit data, synchronous. features: float [12][n] (feature-
+ * major); pred, targets: float [3][n]. ht_mlp_train_step: one batch of n (a
+ * multiple of 64, <= max_batch), forward + backward; apply = 1 also takes
+ * the Adam step with learning rate lr; *loss (optional) = the batch loss. */
+int ht_mlp_forward(ht_cache *cache, ht_buf *features, uint32_t n, ht_buf *pred);
+int ht_mlp_train_step(ht_cache *cache, ht_buf *features, ht_buf *targets, uint32_t n, float lr, int apply,
+                      float *loss);
+
+/* One cached pass (asynchronous): the scene program's render_cached kernel
+ * (paths end into the cache at their query_hop-th hit, 0 = the first; one
+ * pixel in train_every, by a hash of pixel and pass_index, runs unbiased to
+ * full depth and records a training sample), then the MLP inference that
+ * adds throughput x prediction to each queried pixel. canvas gains one pass.
+ * Needs a program whose scene is a vector of mapped covered objects. */
+int ht_render_cached(ht_prog *prog, ht_cache *cache, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas,
+                     ht_buf *seeds, uint32_t pass_index, uint32_t train_every, uint32_t query_hop);
+/* `steps` Adam steps of `batch` samples on the training records of the last
+ * pass (sample i of step k is record (k * batch + i) mod count). Asynchronous. */
+int ht_cache_train(ht_cache *cache, uint32_t steps, uint32_t batch, float lr);
+
 /* Per-pixel seed buffer. Replaces Canvas::new's seed fill
  * (processing/src/canvas.rs:73-84).
  * mode 0: rand 0.8 SmallRng::seed_from_u64(seed).fill(&mut [u32]) on 64-bit

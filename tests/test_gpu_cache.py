@@ -1,0 +1,185 @@
+"""The radiance cache on the GPU, through the C ABI: MLP forward, gradients
+and Adam against the CPU oracle (oracle/mlp.py, BF16 rounding emulated),
+training on a smooth function, the cached render's queues, and the ablation:
+what the trained network gains over the same method with an untrained one
+and over plain path tracing, in RMSE against an FP64 image."""
+import numpy as np
+import pytest
+
+from oracle import mlp as om
+from hypertrace_amd import scenes
+from hypertrace_amd.native import HtError
+from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
+from hypertrace_amd.radiance_cache import CachedRender, RadianceCache, init_params, pack_params, unpack_params
+from hypertrace_amd.types import X32_F32, X32_F64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0, X32_F32)
+    yield c
+    c.close()
+
+
+def _params(seed):
+    p = {k: np.asarray(v, np.float64) for k, v in unpack_params(init_params(seed)).items()}
+    rng = np.random.default_rng(seed + 1)
+    for l in range(6):
+        p[f"b{l}"] = rng.normal(0.0, 0.1, p[f"b{l}"].shape)
+    flat = pack_params(p)
+    return unpack_params(flat.astype(np.float64)), flat
+
+
+def _features(rng, n):
+    f = rng.uniform(-1, 1, (n, 12))
+    f[:, 9:] = rng.uniform(0, 1, (n, 3))
+    return f.astype(np.float32)
+
+
+def test_forward_matches_oracle(ctx):
+    rng = np.random.default_rng(21)
+    p, flat = _params(11)
+    cache = RadianceCache(ctx, 4096, 4096, 1024)
+    cache.set_params(flat)
+    feat = _features(rng, 1000)
+    got = cache.forward(feat)
+    want = om.forward(p, feat, True)["out"]
+    ok = np.abs(got - want) <= 1e-2 * np.abs(want) + 2e-3
+    print("forward: max |diff| %.3g, within tolerance %.4f" % (np.abs(got - want).max(), ok.mean()))
+    assert ok.mean() >= 0.995
+
+
+def test_gradients_match_oracle(ctx):
+    rng = np.random.default_rng(22)
+    p, flat = _params(12)
+    cache = RadianceCache(ctx, 4096, 4096, 1024)
+    cache.set_params(flat)
+    feat = _features(rng, 1024)
+    target = rng.uniform(0, 2, (1024, 3)).astype(np.float32)
+    loss = cache.train_step(feat, target, apply=False)
+    g = unpack_params(cache.grads())
+    fwd = om.forward(p, feat, True)
+    go = om.backward(p, fwd, target.astype(np.float64), True)
+    assert abs(loss - om.loss(fwd["out"], target)) <= 1e-2 * om.loss(fwd["out"], target)
+    for name in om.NAMES:
+        a, b = g[name].astype(np.float64).ravel(), go[name].ravel()
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+        print(f"{name}: rel {rel:.3g} cos {cos:.6f}")
+        assert rel < 3e-2 and cos > 0.999, name
+    assert np.array_equal(cache.params(), flat)  # apply=False leaves the parameters
+
+
+def test_adam_steps_match_oracle(ctx):
+    rng = np.random.default_rng(23)
+    p, flat = _params(13)
+    cache = RadianceCache(ctx, 4096, 4096, 1024)
+    cache.set_params(flat)
+    m = {k: np.zeros_like(v) for k, v in p.items()}
+    v = {k: np.zeros_like(x) for k, x in p.items()}
+    for step in (1, 2):
+        feat = _features(rng, 512)
+        target = rng.uniform(0, 2, (512, 3)).astype(np.float32)
+        cache.train_step(feat, target, lr=1e-3, apply=True)
+        g = {k: x.astype(np.float64) for k, x in unpack_params(cache.grads()).items()}
+        om.adam(p, m, v, g, step, 1e-3)
+        got = unpack_params(cache.params())
+        for name in om.NAMES:
+            assert np.allclose(got[name], p[name], rtol=1e-5, atol=1e-7), (step, name)
+        p = {k: x.astype(np.float64) for k, x in got.items()}  # continue from the GPU's FP32 values
+
+
+def test_training_fits_a_smooth_function(ctx):
+    rng = np.random.default_rng(24)
+    cache = RadianceCache(ctx, 4096, 4096, 2048)
+
+    def f(x):
+        r = 0.6 + 0.4 * np.sin(2.0 * x[:, 0] + x[:, 4]) * x[:, 9]
+        g = 0.5 + 0.3 * np.cos(3.0 * x[:, 1] - x[:, 7])
+        b = 0.4 + 0.3 * x[:, 2] * x[:, 11]
+        return np.stack([r, g, b], 1).astype(np.float32)
+
+    losses = []
+    for _ in range(300):
+        x = _features(rng, 2048)
+        losses.append(cache.train_step(x, f(x), lr=3e-3))
+    test = _features(rng, 2048)
+    err = np.abs(cache.forward(test) - f(test)).mean()
+    print(f"loss {losses[0]:.4f} -> {np.mean(losses[-20:]):.5f}, mean abs error {err:.4f}")
+    assert np.mean(losses[-20:]) < 0.05 * losses[0]
+    assert err < 0.12  # values span 0.2 .. 1.0
+
+
+def test_cached_render_queues(ctx):
+    wl = scenes.hyperbolic_cells(96, 72, 1)
+    buf = EntityBuffer(ctx, wl.scene)
+    cache = RadianceCache(ctx, 96 * 72, 65536, 1024)
+    cr = CachedRender(ctx, wl.scene, cache, steps_per_pass=1, batch=1024)
+    canvas = Canvas(ctx, (96, 72), wl.seed_mode, wl.seed)
+    cr.render_pass(buf, canvas)
+    ctx.sync()
+    q, r = cache.counts()
+    print("queries", q, "records", r)
+    assert 0 < q < 96 * 72 and 0 < r < 96 * 72 // 4 and q + r <= 96 * 72
+    img = canvas.load_image()
+    assert np.all(np.isfinite(img)) and np.all(img[..., 3] == 1.0)
+
+
+def test_cached_render_needs_a_supported_scene(ctx):
+    wl = scenes.reference_hyperbolic_example(32, 24)
+    buf = EntityBuffer(ctx, wl.scene)
+    cache = RadianceCache(ctx, 32 * 24, 1024, 64)
+    cr = CachedRender(ctx, wl.scene, cache)
+    canvas = Canvas(ctx, (32, 24), wl.seed_mode, wl.seed)
+    with pytest.raises(HtError, match="radiance-cache kernel"):
+        cr.render_pass(buf, canvas)
+
+
+def _gamma(c):
+    img = c[..., :3].astype(np.float64) / c[..., 3:4]
+    return np.clip(np.maximum(img, 0.0) ** (1 / 2.2), 0.0, 1.0)
+
+
+def _rmse(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)))
+
+
+def test_ablation_trained_cache(ctx):
+    """Same spp, RMSE against an FP64 1024-spp image of the same frame: the
+    cached method with its trained network, with an untrained network, and
+    plain path tracing. Measured on MI355X: plain 0.0333, untrained 0.0501,
+    trained 0.0321 — in this scene only ~12 % of paths reach their third hit,
+    so the cache changes only those pixels."""
+    W, H, SPP = 128, 96, 16
+    wl = scenes.hyperbolic_cells(W, H, SPP)
+    with Context(0, X32_F64) as c64:
+        r64 = Render(c64, wl.scene)
+        b64 = EntityBuffer(c64, wl.scene)
+        ref_c = Canvas(c64, (W, H), 1, wl.seed ^ 0x0BAD5EED)
+        for _ in range(4):
+            r64.render_passes(b64, ref_c, 256)
+        ref = _gamma(ref_c.load_image())
+    buf = EntityBuffer(ctx, wl.scene)
+    plain = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    Render(ctx, wl.scene).render_passes(buf, plain, SPP)
+    rmse_plain = _rmse(_gamma(plain.load_image()), ref)
+
+    def cached(train_passes):
+        cache = RadianceCache(ctx, W * H, 65536, 1024, seed=5)
+        cr = CachedRender(ctx, wl.scene, cache, steps_per_pass=4, batch=1024, lr=1e-3)
+        warm = Canvas(ctx, (W, H), 1, wl.seed ^ 0x5EED)
+        for _ in range(train_passes):
+            cr.render_pass(buf, warm)
+        canvas = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        for _ in range(SPP):
+            cr.render_pass(buf, canvas, train=train_passes > 0)
+        return _rmse(_gamma(canvas.load_image()), ref)
+
+    rmse_untrained = cached(0)
+    rmse_trained = cached(256)
+    print(f"RMSE at {SPP} spp: plain {rmse_plain:.4f}, cache untrained {rmse_untrained:.4f}, "
+          f"cache trained {rmse_trained:.4f}")
+    assert rmse_trained < 0.8 * rmse_untrained
+    assert rmse_trained < 1.02 * rmse_plain
