@@ -103,13 +103,16 @@ def rmse(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)))
 
 
-def time_to_rmse(ctx, wl, buf, chosen, plain_render, args):
+def fp64_reference(wl, oracle_spp):
+    """FP64 image of the workload at oracle_spp from two independent halves:
+    (gamma image, its own noise RMSE, seconds). The halves differ by twice
+    the variance of their mean."""
     from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
     from hypertrace_amd.types import X32_F64
 
     W, H = wl.width, wl.height
     t_or = time.perf_counter()
-    half = max(1, args.oracle_spp // 2)
+    half = max(1, oracle_spp // 2)
     halves = []
     with Context(0, X32_F64) as ctx64:
         r64 = Render(ctx64, wl.scene)
@@ -122,10 +125,18 @@ def time_to_rmse(ctx, wl, buf, chosen, plain_render, args):
                 r64.render_passes(b64, c, n)
                 done += n
             halves.append(c.load_image())
-    oracle_s = time.perf_counter() - t_or
     ref = gamma_image(halves[0] + halves[1])
-    # noise of the 4096-spp oracle: the two halves differ by twice its variance
-    oracle_noise = rmse(gamma_image(halves[0]), gamma_image(halves[1])) / 2.0
+    noise = rmse(gamma_image(halves[0]), gamma_image(halves[1])) / 2.0
+    return ref, noise, time.perf_counter() - t_or, 2 * half
+
+
+def time_to_rmse(ctx, wl, buf, chosen, plain_render, args, reference=None):
+    W, H = wl.width, wl.height
+    from hypertrace_amd.processing import Canvas
+
+    ref, oracle_noise, oracle_s, oracle_spp = reference if reference is not None else fp64_reference(
+        wl, args.oracle_spp)
+    half = oracle_spp // 2
 
     # spp at which the FP32 image first reaches the target: steps of 16
     # passes, then pass by pass inside the last step.
@@ -176,6 +187,62 @@ def time_to_rmse(ctx, wl, buf, chosen, plain_render, args):
     return res
 
 
+def cache_leg(args):
+    """The radiance-cache method on the 1920x1080 hyperbolic cell scene with
+    256 primitives: time until RMSE <= target against an FP64 4096-spp image,
+    next to the chosen bit-exact form and the plain per-pass kernel. The
+    cached passes are timed with their training (4 Adam steps of 16384
+    suffixes per pass); image readbacks for the RMSE checks are not."""
+    from hypertrace_amd import scenes
+    from hypertrace_amd.maps import compose_on_device
+    from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
+    from hypertrace_amd.radiance_cache import CachedRender, RadianceCache
+    from hypertrace_amd.types import X32_F32
+
+    wl = scenes.hyperbolic_cells(1920, 1080, 1, primitives=256)
+    W, H = wl.width, wl.height
+    reference = fp64_reference(wl, args.oracle_spp)
+    ref = reference[0]
+    with Context(0, X32_F32) as ctx:
+        buf = EntityBuffer(ctx, wl.scene)
+        compose_on_device(ctx, buf, wl.scene, wl.word_maps)
+        chosen = Render(ctx, wl.scene)
+        plain = None if args.no_plain else Render(ctx, wl.scene, fp_options=NO_CULL)
+        base = time_to_rmse(ctx, wl, buf, chosen, plain, args, reference)
+        cache = RadianceCache(ctx, W * H, 65536, 16384)
+        cr = CachedRender(ctx, wl.scene, cache, query_hop=2, train_every=16, steps_per_pass=4, batch=16384,
+                          lr=1e-3)
+        canvas = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        t, spp, curve, reached, check = 0.0, 0, [], None, 4
+        while spp < args.cache_max_spp:
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(check):
+                cr.render_pass(buf, canvas)
+            ctx.sync()
+            t += time.perf_counter() - t0
+            spp += check
+            e = rmse(gamma_image(canvas.load_image()), ref)
+            curve.append((spp, round(e, 6)))
+            if e <= args.target_rmse:
+                reached = spp
+                break
+        queries, records = cache.counts()
+    out = {"workload": "hyperbolic_cells 1920x1080, 256 primitives, 8 hops", "target": args.target_rmse,
+           "oracle_spp": reference[3], "oracle_noise_rmse": reference[1], "oracle_render_s": reference[2],
+           "bit_exact_forms": {k: base.get(k) for k in ("reached", "spp", "chosen_s", "plain_s", "speedup")},
+           "cached": {"reached": reached is not None, "spp": reached if reached is not None else spp,
+                      "seconds": t, "ms_per_pass": t * 1e3 / max(spp, 1), "curve": curve[::4] + curve[-1:],
+                      "queries_last_pass": queries, "training_records_last_pass": records,
+                      "method": "paths end into the MLP cache at their third hit; 1/16 of pixels per pass "
+                                "run unbiased to full depth and train it (4 Adam steps x 16384 per pass)"}}
+    if reached is not None and base.get("plain_s"):
+        out["cached"]["speedup_vs_plain"] = base["plain_s"] / t
+    if reached is not None and base.get("chosen_s"):
+        out["cached"]["speedup_vs_chosen"] = base["chosen_s"] / t
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +250,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="time only the chosen form")
+    ap.add_argument("--no-cache", action="store_true", help="skip the 1080p radiance-cache leg")
+    ap.add_argument("--cache-max-spp", type=int, default=1024)
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
@@ -305,6 +374,11 @@ def main():
         "roofline": roofline,
     }
     out.update(extra)
+    if not args.no_cache and not args.no_rmse:
+        try:
+            out["radiance_cache_1080p"] = cache_leg(args)
+        except Exception as e:  # the headline stays valid if this leg fails
+            out["radiance_cache_1080p"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, W, H, seeds0)
     print(json.dumps(out))

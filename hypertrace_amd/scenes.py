@@ -118,10 +118,13 @@ CELL_GENERATORS = (
 
 
 def hyperbolic_cells(width: int = 640, height: int = 480, spp: int = 64, cells: int = 64,
-                     hops: int = 8) -> Workload:
+                     hops: int = 8, primitives: Optional[int] = None) -> Workload:
     """The hyperbolic cell scene; every primitive's map is the FP64 product
     word * base with det = 1 reprojection after each factor
-    (maps.compose_word, bit-identical to the GPU composition)."""
+    (maps.compose_word, bit-identical to the GPU composition). `primitives`
+    keeps only the first that many (the cells needed are generated)."""
+    if primitives is not None:
+        cells = (primitives + 2) // 3
     from .maps import WordMaps, compose_word
 
     rng = _rng(2)
@@ -146,6 +149,9 @@ def hyperbolic_cells(width: int = 640, height: int = 480, spp: int = 64, cells: 
             albedo = rng.uniform(0.3, 0.9, 3)
             material = Mixture([(Colored(Lambertian(), albedo), 0.7), (Specular(), 0.3)])
             objs.append(Mapped(Covered(ShapeChoice(choice, kind), material), m))
+    if primitives is not None:
+        objs = objs[:primitives]
+        wm = WordMaps(wm.gens, wm.words[:primitives], wm.posts[:primitives], wm.targets[:primitives])
     view = Mapped(PointView(1.0), hy_zshift(math.log(2.0)))
     bg = ConstBg((1.0, 1.0, 1.0))
     scene = SceneImpl(HY, view, ObjectVec(objs), bg, hops)
