@@ -52,15 +52,19 @@ def standard_scenes():
         scenes.hyperbolic_cells().scene,
         scenes.reference_euclidean_example().scene,
         scenes.reference_hyperbolic_example().scene,
+        scenes.sdf_shapes().scene,
+        scenes.sdf_shapes(96, 72, 1, shapes=48, extent=4.0, camera_z=12.0).scene,
     ]
 
 
 def precompile_programs(verbose: bool = False) -> None:
-    from .processing import FAST_FP, NO_CULL_FP, precompile
+    from .processing import FAST_FP, NO_CULL_FP, precompile, program_options
     from .types import X32_F32, X32_F64
 
+    rr = program_options(rr_depth=3)
     for scene in standard_scenes():
-        for cfg, fp in ((X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP), (X32_F32, NO_CULL_FP)):
+        extra = [(X32_F32, rr), (X32_F64, rr)] if scene.geo == "Eu" else []
+        for cfg, fp in [(X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP), (X32_F32, NO_CULL_FP)] + extra:
             t = time.time()
             hit = precompile(scene, cfg, fp_options=fp)
             if verbose:

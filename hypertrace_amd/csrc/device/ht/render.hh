@@ -33,10 +33,29 @@ HD PixelSetup pixel_setup(uint x, uint y, uint w, uint h) {
     return p;
 }
 
-// One bounce of scene.inl:25-41. Returns true when the path ends here.
+// Russian roulette (program option HT_RR_DEPTH = d; not in the reference):
+// from the d-th hit on (hop >= d), before the nearest-hit search, the path
+// survives with probability q = min(max(throughput), 0.95) (one uniform) and
+// its throughput is divided by q. Returns true when the path ends.
+HD bool russian_roulette(Context *ctx, LightBase *base) {
+    color3 I = base->intensity;
+    float q = fminf(fmaxf(fmaxf(I.x, I.y), I.z), 0.95f);
+    real u = random_uniform(&ctx->rng);
+    if (u >= (real)q) return true;
+    base->intensity = I / q;
+    return false;
+}
+
+// One bounce of scene.inl:25-41 (`hop` = hits so far). Returns true when the
+// path ends here.
 template <class S>
-HD bool bounce(const u8 *scene, Context *ctx, Light<typename S::Geo> *light, color3 *emission) {
+HD bool bounce(const u8 *scene, Context *ctx, Light<typename S::Geo> *light, color3 *emission, int hop) {
     typedef typename S::Object T;
+#ifdef HT_RR_DEPTH
+    if (hop >= HT_RR_DEPTH && russian_roulette(ctx, &light->base)) return true;
+#else
+    (void)hop;
+#endif
     const u8 *obj = S::object(scene);
     ctx->hasher = hash_init();
     typename T::Cache cache;
@@ -78,7 +97,7 @@ HD void render_pass_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *_
     color3 emission;
     start_path<S>(scene, &ctx, px, &light, &emission);
     for (int i = 0; i < S::HOPS; ++i) {
-        if (bounce<S>(scene, &ctx, &light, &emission)) break;
+        if (bounce<S>(scene, &ctx, &light, &emission, i)) break;
     }
     float4 acc = canvas[idx];
     acc.x += emission.x;
@@ -123,7 +142,7 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
                 hop = 0;
                 start = false;
             }
-            bool end = bounce<S>(scene, &ctx, &light, &emission);
+            bool end = bounce<S>(scene, &ctx, &light, &emission, hop);
             ++hop;
             if (hop >= H) end = true;
             if (end) {

@@ -25,6 +25,7 @@
 //    get the hit state, and maps it back to the parent frame once.
 #pragma once
 #include "entities.hh"
+#include "sdf.hh"
 #include "tiled.hh"
 
 #ifndef HT_BOUNDS_CAP
@@ -59,6 +60,14 @@ HD double cull_margin(double n2) { return 1e-3 + 1e-4 * n2 * n2; }
 template <class S> struct PrimKind { static HD int kind(const u8 *) { return 0; } static HD double radius(const u8 *) { return 0.0; } };
 template <> struct PrimKind<Sphere> { static HD int kind(const u8 *) { return 1; } static HD double radius(const u8 *) { return 1.0; } };
 template <> struct PrimKind<Ball> { static HD int kind(const u8 *) { return 2; } static HD double radius(const u8 *s) { return (double)ld<real>(s); } };
+// Sphere-traced shapes: the march's bounding sphere (bound + 2 eps) plus 2 eps.
+template <class S> struct SdfKind {
+    static HD int kind(const u8 *) { return 2; }
+    static HD double radius(const u8 *s) { return (double)S::bound(s) + 4.0 * (double)SDF_EPS; }
+};
+template <> struct PrimKind<SdfSphere> : SdfKind<SdfSphere> {};
+template <> struct PrimKind<SdfBox> : SdfKind<SdfBox> {};
+template <> struct PrimKind<SdfTorus> : SdfKind<SdfTorus> {};
 template <> struct PrimKind<Cube> { static HD int kind(const u8 *) { return 3; } static HD double radius(const u8 *) { return 1.7320508075688772; } };
 template <> struct PrimKind<Plane<Hy>> { static HD int kind(const u8 *) { return 4; } static HD double radius(const u8 *) { return 1.0; } };
 template <> struct PrimKind<Horosphere> { static HD int kind(const u8 *) { return 5; } static HD double radius(const u8 *) { return 1.0; } };

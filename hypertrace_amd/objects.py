@@ -375,6 +375,58 @@ class Ball(Entity):
         return ["ball", _hr(self.r, cfg)]
 
 
+class SdfSphere(Entity):
+    """Sphere-traced sphere of radius r (not in the reference: its shapes are
+    analytic; device ht/sdf.hh)."""
+
+    def __init__(self, r=1.0):
+        self.r = float(r)
+
+    def fields(self):
+        return [_Real(self.r)]
+
+    def cpp(self, geo):
+        assert geo == EU
+        return "ht::SdfSphere"
+
+    def spec(self, geo, cfg):
+        return ["sdfsphere", _hr(self.r, cfg)]
+
+
+class SdfBox(Entity):
+    """Sphere-traced box with half extents h (real3)."""
+
+    def __init__(self, h=(1.0, 1.0, 1.0)):
+        self.h = [float(x) for x in h]
+
+    def fields(self):
+        return [_Real3(self.h)]
+
+    def cpp(self, geo):
+        assert geo == EU
+        return "ht::SdfBox"
+
+    def spec(self, geo, cfg):
+        return ["sdfbox"] + [_hr(x, cfg) for x in self.h]
+
+
+class SdfTorus(Entity):
+    """Sphere-traced torus around the local y axis: major radius R, minor r."""
+
+    def __init__(self, big=1.0, small=0.25):
+        self.big, self.small = float(big), float(small)
+
+    def fields(self):
+        return [_Real(self.big), _Real(self.small)]
+
+    def cpp(self, geo):
+        assert geo == EU
+        return "ht::SdfTorus"
+
+    def spec(self, geo, cfg):
+        return ["sdftorus", _hr(self.big, cfg), _hr(self.small, cfg)]
+
+
 class Cube(_Zst):
     """Unit cube [-1, 1]^3 (shape/eu/cube.cc)."""
 

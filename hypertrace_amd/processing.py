@@ -209,6 +209,18 @@ class Program:
             pass
 
 
+def program_options(fast: bool = False, rr_depth: Optional[int] = None, no_cull: bool = False) -> str:
+    """Program build options: strict (default) or fast FP32; Russian roulette
+    from the rr_depth-th hit (a different method: it draws one more uniform
+    per hit from there on); the element-by-element scan (comparisons)."""
+    opts = FAST_FP if fast else STRICT_FP
+    if rr_depth is not None:
+        opts += f" -DHT_RR_DEPTH={int(rr_depth)}"
+    if no_cull:
+        opts += " -DHT_NO_CULL=1"
+    return opts
+
+
 # Program FP modes (see include/hypertrace_hip.h, ht_prog_build_ex).
 STRICT_FP = "-ffp-contract=off -DHT_STRICT_FP=1"  # the default
 FAST_FP = "-fno-hip-fp32-correctly-rounded-divide-sqrt"

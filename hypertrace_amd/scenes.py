@@ -158,6 +158,48 @@ def hyperbolic_cells(width: int = 640, height: int = 480, spp: int = 64, cells: 
     return Workload("hyperbolic_cells", scene, width, height, spp, 1, MASTER_SEED ^ 2, wm)
 
 
+def sdf_shapes(width: int = 1920, height: int = 1080, spp: int = 256, shapes: int = 1024, hops: int = 16,
+               extent: float = 10.0, camera_z: float = 26.0) -> Workload:
+    """The sphere-traced scene: `shapes` SDF primitives (sphere : box : torus =
+    1 : 1 : 1 by index mod 3) with sizes s ~ U[0.1, 1] (sphere radius s; box
+    half extents s * U[0.5, 1]^3; torus R = s, r = s * U[0.2, 0.4]), uniformly
+    random rotations, centres U[-extent, extent]^3; materials diffuse (albedo
+    U[0.3, 0.9]^3) / mirror / glass (IOR 1.5) with probabilities 0.6 / 0.25 /
+    0.15 as one-hot mixtures, 2 % of the shapes emissive (radiance 4); gradient
+    sky; pinhole at (0, 0, camera_z) looking down -z, 40 degree vertical FOV.
+    Russian roulette from the 3rd hit is a program option
+    (processing.program_options(rr_depth=3))."""
+    from .objects import Refractive, SdfBox, SdfSphere, SdfTorus
+
+    rng = _rng(4)
+    objs = []
+    for i in range(shapes):
+        kind = i % 3
+        size = rng.uniform(0.1, 1.0)
+        variants = [SdfSphere(1.0), SdfBox((1.0, 1.0, 1.0)), SdfTorus(1.0, 0.25)]
+        if kind == 0:
+            variants[0] = SdfSphere(size)
+        elif kind == 1:
+            variants[1] = SdfBox(size * rng.uniform(0.5, 1.0, 3))
+        else:
+            variants[2] = SdfTorus(size, size * rng.uniform(0.2, 0.4))
+        q = rng.normal(size=4)
+        q = q / np.linalg.norm(q)
+        centre = rng.uniform(-extent, extent, 3)
+        u = rng.uniform()
+        onehot = (1.0, 0.0, 0.0) if u < 0.6 else (0.0, 1.0, 0.0) if u < 0.85 else (0.0, 0.0, 1.0)
+        albedo = rng.uniform(0.3, 0.9, 3)
+        emission = (4.0, 4.0, 4.0) if rng.uniform() < 0.02 else (0.0, 0.0, 0.0)
+        material = Emissive(Mixture([(Colored(Lambertian(), albedo), onehot[0]), (Specular(), onehot[1]),
+                                     (Refractive(1.5), onehot[2])]), emission)
+        objs.append(Mapped(Covered(ShapeChoice(variants, kind), material), Homogenous3(centre, q)))
+    fov = math.tan(math.radians(20.0))
+    view = Mapped(PointView(fov), Homogenous3((0.0, 0.0, camera_z), (1, 0, 0, 0)))
+    bg = GradBg((0.0, 1.0, 0.0), [(1.0, 1.0, 1.0), (0.1, 0.1, 0.12)], 1.0)
+    scene = SceneImpl(EU, view, ObjectVec(objs), bg, hops)
+    return Workload("sdf_shapes", scene, width, height, spp, 1, MASTER_SEED ^ 4)
+
+
 def _unpack_color(rgb: int):
     """0xRRGGBB -> linear colour, as main/examples/hy.rs:27-35 (gamma 2.2)."""
     return [(((rgb >> s) & 0xFF) / 255.0) ** 2.2 for s in (16, 8, 0)]

@@ -48,6 +48,8 @@ def lib():
         l.oracle_look_at.argtypes = [c_void_p, c_void_p]
         l.oracle_shape_detect.restype = c_double
         l.oracle_shape_detect.argtypes = [c_int, c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+        l.oracle_sdf_detect.restype = c_double
+        l.oracle_sdf_detect.argtypes = [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
         l.oracle_post.argtypes = [c_void_p, c_size_t, c_uint32, c_float, c_void_p, c_void_p]
         _lib = l
     return _lib
@@ -109,6 +111,18 @@ def shape_detect(kind: int, start, direction, repeat: bool = False, param: float
     os_, od, on = np.zeros(4), np.zeros(4), np.zeros(4)
     dist = lib().oracle_shape_detect(kind, param, int(repeat), _p(s), _p(d), _p(os_), _p(od), _p(on))
     return float(dist), os_, od, on
+
+
+def sdf_detect(kind: int, params, start, direction, repeat: bool = False):
+    """Sphere-traced shape in FP64 (0 sphere r, 1 box h, 2 torus R r):
+    (distance or -1, hit point, normal)."""
+    pr = np.zeros(3, np.float64)
+    pr[: len(params)] = params
+    s = np.asarray(start, np.float64).copy()
+    d = np.asarray(direction, np.float64).copy()
+    os_, on = np.zeros(3), np.zeros(3)
+    dist = lib().oracle_sdf_detect(kind, _p(pr), int(repeat), _p(s), _p(d), _p(os_), _p(on))
+    return float(dist), os_, on
 
 
 def post(canvas: np.ndarray, passes: int, gamma: float):

@@ -381,6 +381,79 @@ template <class R, class F> struct PlaneEu : Shape<R, F, GeoEu<R>> {
 
 template <class R> R sgn(R a) { return R((R(0) < a) - (a < R(0))); }
 
+// Sphere-traced signed-distance shapes (not in the reference; the method of
+// hypertrace_amd/csrc/device/ht/sdf.hh): kind 0 sphere {r}, 1 box {hx, hy, hz},
+// 2 torus around y {R, r}. The march runs inside the bounding sphere of radius
+// bound + 2 eps, stepping |f| until |f| < eps, at most 128 evaluations; a ray
+// leaving the primitive it just hit first steps max(|f|, eps) until |f| >= 2 eps;
+// the normal is the normalised tetrahedron gradient estimate (h = 4 eps).
+template <class R> struct SdfConsts;
+template <> struct SdfConsts<float> { static float eps() { return 1e-4f; } };
+template <> struct SdfConsts<double> { static double eps() { return 1e-4; } };
+template <> struct SdfConsts<Cnt> { static Cnt eps() { return Cnt(1e-4f); } };
+
+template <class R, class F> struct SdfEu : Shape<R, F, GeoEu<R>> {
+    int kind = 0;
+    R a = R(0), b = R(0), c = R(0);
+    R eval(V3<R> x) const {
+        if (kind == 0) return length(x) - a;
+        if (kind == 1) {
+            V3<R> q = {fabs_(x.x) - a, fabs_(x.y) - b, fabs_(x.z) - c};
+            V3<R> qp = {fmax_(q.x, R(0)), fmax_(q.y, R(0)), fmax_(q.z, R(0))};
+            return length(qp) + fmin_(fmax_(q.x, fmax_(q.y, q.z)), R(0));
+        }
+        R qx = fsqrt(x.x * x.x + x.z * x.z) - a;
+        return fsqrt(qx * qx + x.y * x.y) - b;
+    }
+    R bound() const {
+        if (kind == 0) return a;
+        if (kind == 1) return length(V3<R>{a, b, c});
+        return a + b;
+    }
+    R detect(Ctx &ctx, V3<R> &normal, Light<R, F, GeoEu<R>> &light) const override {
+        const R EPS = SdfConsts<R>::eps();
+        bool repeat = ctx.repeat();
+        V3<R> p = light.ray.start, d = light.ray.dir;
+        R rb = bound() + R(2) * EPS;
+        R bb = -dot(d, p);
+        V3<R> g = p + d * bb;
+        R disc = rb * rb - length2(g);
+        if (disc < R(0)) return R(-1);
+        R s = fsqrt(disc);
+        R t1 = bb + s;
+        if (t1 < R(0)) return R(-1);
+        R t = fmax_(bb - s, R(0));
+        bool leaving = repeat, hit = false;
+        for (int k = 0; k < 128; ++k) {
+            R av = fabs_(eval(p + d * t));
+            if (leaving) {
+                if (av >= R(2) * EPS) {
+                    leaving = false;
+                } else {
+                    t += fmax_(av, EPS);
+                    if (t > t1) break;
+                    continue;
+                }
+            }
+            if (av < EPS) {
+                hit = true;
+                break;
+            }
+            t += av;
+            if (t > t1) break;
+        }
+        if (!hit) return R(-1);
+        V3<R> x = p + d * t;
+        R h = R(4) * EPS;
+        R f0 = eval(x + V3<R>{h, -h, -h}), f1 = eval(x + V3<R>{-h, -h, h});
+        R f2 = eval(x + V3<R>{-h, h, -h}), f3 = eval(x + V3<R>{h, h, h});
+        V3<R> n = {f0 - f1 - f2 + f3, -f0 - f1 + f2 + f3, -f0 + f1 - f2 + f3};
+        normal = normalize(n);
+        light.ray.start = x;
+        return t;
+    }
+};
+
 template <class R, class F> struct CubeEu : Shape<R, F, GeoEu<R>> {
     // shape/eu/cube.cc:4-21
     static R nearest(V3<R> n, V3<R> &normal) {
@@ -1174,6 +1247,7 @@ template <class R, class F> struct GradBg : Background<R, F, GeoEu<R>> {
 // ------------------------------------------------------------ scene + parser
 template <class R, class F, class G> struct Scene {
     int hops = 0;
+    int rr_depth = -1;  // Russian roulette from this hit on (-1: none)
     std::unique_ptr<View<R, G>> view;
     std::unique_ptr<Background<R, F, G>> bg;
     std::unique_ptr<Object<R, F, G>> object;
@@ -1186,6 +1260,14 @@ template <class R, class F, class G> struct Scene {
         C3<F> e = {F(0.0f), F(0.0f), F(0.0f)};
         ctx.prev_hash = 0xFFFFFFFFu;
         for (int i = 0; i < hops; ++i) {
+            if (rr_depth >= 0 && i >= rr_depth) {
+                // survive with q = min(max(throughput), 0.95), throughput /= q
+                C3<F> I = light.intensity;
+                F q = fmin_(fmax_(fmax_(I.x, I.y), I.z), F(0.95f));
+                R u = uniform<R>(ctx.rng);
+                if (u >= R(tofloat(q))) break;
+                light.intensity = {I.x / q, I.y / q, I.z / q};
+            }
             ctx.hasher = 0;
             Cache<R, G> cache{};
             R dist = object->detect(ctx, cache, light);
@@ -1349,6 +1431,14 @@ template <class R, class F> struct ShapeParse {
         if (k == "sphere") return std::unique_ptr<Shape<R, F, GeoEu<R>>>(new SphereEu<R, F>(R(1), true));
         if (k == "ball") return std::unique_ptr<Shape<R, F, GeoEu<R>>>(new SphereEu<R, F>(R(p.tk.num()), false));
         if (k == "cube") return std::unique_ptr<Shape<R, F, GeoEu<R>>>(new CubeEu<R, F>());
+        if (k == "sdfsphere" || k == "sdfbox" || k == "sdftorus") {
+            auto sh = new SdfEu<R, F>();
+            sh->kind = k == "sdfsphere" ? 0 : k == "sdfbox" ? 1 : 2;
+            sh->a = R(p.tk.num());
+            if (sh->kind >= 1) sh->b = R(p.tk.num());
+            if (sh->kind == 1) sh->c = R(p.tk.num());
+            return std::unique_ptr<Shape<R, F, GeoEu<R>>>(sh);
+        }
         if (k == "plane") return std::unique_ptr<Shape<R, F, GeoEu<R>>>(new PlaneEu<R, F>());
         return nullptr;
     }
@@ -1428,8 +1518,9 @@ template <class R, class F, class G> std::unique_ptr<Background<R, F, G>> Parser
     return b;
 }
 
-template <class R, class F, class G> std::unique_ptr<Scene<R, F, G>> parse_scene(Tokens &tk) {
+template <class R, class F, class G> std::unique_ptr<Scene<R, F, G>> parse_scene(Tokens &tk, int rr_depth) {
     auto s = std::unique_ptr<Scene<R, F, G>>(new Scene<R, F, G>());
+    s->rr_depth = rr_depth;
     s->hops = (int)tk.integer();
     Parser<R, F, G> p(tk, std::is_same<G, GeoHy<R>>::value);
     s->view = p.view();
@@ -1481,15 +1572,22 @@ uint64_t render_impl(const Scene<R, F, G> &scene, uint32_t w, uint32_t h, uint32
 template <class R, class F>
 uint64_t render_any(const char *spec, uint32_t w, uint32_t h, uint32_t passes, float *canvas, uint32_t *seeds,
                     uint32_t x0, uint32_t y0, uint32_t cw, uint32_t ch, int threads, uint64_t *flops64 = nullptr) {
+    // optional prefix "rr <depth>": Russian roulette from that hit on
     Tokens tk(spec);
-    tk.expect("scene");
+    int rr = -1;
+    std::string first = tk.next();
+    if (first == "rr") {
+        rr = (int)tk.integer();
+        first = tk.next();
+    }
+    if (first != "scene") throw std::runtime_error("spec must start with 'scene'");
     std::string geo = tk.next();
     if (geo == "eu") {
-        auto s = parse_scene<R, F, GeoEu<R>>(tk);
+        auto s = parse_scene<R, F, GeoEu<R>>(tk, rr);
         return render_impl(*s, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads, flops64);
     }
     if (geo == "hy") {
-        auto s = parse_scene<R, F, GeoHy<R>>(tk);
+        auto s = parse_scene<R, F, GeoHy<R>>(tk, rr);
         return render_impl(*s, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads, flops64);
     }
     throw std::runtime_error("unknown geometry " + geo);
@@ -1657,6 +1755,31 @@ double oracle_shape_detect(int kind, double param, int repeat, const double *sta
     std::memcpy(out_start, s, sizeof s);
     std::memcpy(out_dir, v, sizeof v);
     std::memcpy(out_normal, nn, sizeof nn);
+    return d;
+}
+
+// Sphere-traced shape in FP64: kind 0 sphere (p0), 1 box (p0, p1, p2), 2 torus (p0, p1).
+double oracle_sdf_detect(int kind, const double *params, int repeat, const double *start, const double *dir,
+                         double *out_start, double *out_normal) {
+    Ctx ctx{};
+    ctx.hasher = 7;
+    ctx.prev_hash = repeat ? 7 : 8;
+    SdfEu<double, float> sh;
+    sh.kind = kind;
+    sh.a = params[0];
+    sh.b = params[1];
+    sh.c = params[2];
+    Light<double, float, GeoEu<double>> l{};
+    l.ray.start = {start[0], start[1], start[2]};
+    l.ray.dir = {dir[0], dir[1], dir[2]};
+    V3<double> n{};
+    double d = sh.detect(ctx, n, l);
+    out_start[0] = l.ray.start.x;
+    out_start[1] = l.ray.start.y;
+    out_start[2] = l.ray.start.z;
+    out_normal[0] = n.x;
+    out_normal[1] = n.y;
+    out_normal[2] = n.z;
     return d;
 }
 

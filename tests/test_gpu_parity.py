@@ -12,7 +12,7 @@ import pytest
 import oracle
 from hypertrace_amd import scenes
 from hypertrace_amd.processing import (NO_CULL_FP, Canvas, Context, EntityBuffer, Extractor, GammaFilter, ImageBuffer, Packer,
-                                       Pipeline, Render, seed_fill)
+                                       Pipeline, Render, program_options, seed_fill)
 from hypertrace_amd.types import X32_F32, X32_F64
 
 pytestmark = pytest.mark.gpu
@@ -55,6 +55,8 @@ WORKLOADS = {
     "hy_cells": lambda: scenes.hyperbolic_cells(96, 72),
     "eu_example": lambda: scenes.reference_euclidean_example(96, 72),
     "hy_example": lambda: scenes.reference_hyperbolic_example(96, 72),
+    "sdf": lambda: scenes.sdf_shapes(96, 72, 1, shapes=48, extent=4.0, camera_z=12.0),
+    "sdf_1024": lambda: scenes.sdf_shapes(320, 240, 1),
 }
 
 
@@ -72,7 +74,7 @@ def test_single_pass_parity_f32(ctx, name):
     assert (got_seeds[ok] == ref_seeds[ok]).mean() >= 0.999
 
 
-@pytest.mark.parametrize("name", ["eu_unit", "hy_cells", "eu_example", "hy_example"])
+@pytest.mark.parametrize("name", ["eu_unit", "hy_cells", "eu_example", "hy_example", "sdf"])
 def test_single_pass_parity_f64(ctx64, name):
     wl = WORKLOADS[name]()
     print(f"case {name} f64", flush=True)
@@ -97,6 +99,23 @@ def test_render_passes_identical_to_repeated_render(ctx, name):
     assert canvas.passes == k
     np.testing.assert_array_equal(img_a.view(np.uint32), img_b.view(np.uint32))
     np.testing.assert_array_equal(seeds_a, seeds_b)
+
+
+@pytest.mark.parametrize("name", ["sdf", "eu_ball"])
+def test_russian_roulette_parity(ctx, name):
+    """Program option HT_RR_DEPTH=3 against the oracle's `rr 3` scene prefix:
+    same survival draws and throughput rescaling, per pixel."""
+    wl = WORKLOADS[name]()
+    print(f"case {name} rr", flush=True)
+    render = Render(ctx, wl.scene, fp_options=program_options(rr_depth=3))
+    buf = EntityBuffer(ctx, wl.scene)
+    canvas = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
+    seeds0 = canvas.load_seeds().copy()
+    render.render_passes(buf, canvas, 2)
+    got = canvas.load_image()
+    ref, _, _ = oracle.render("rr 3 " + wl.scene.spec_text(X32_F32), wl.width, wl.height, 2, seeds0)
+    ok = within(got, ref)
+    assert ok.mean() >= 0.995, f"{name}: only {ok.mean():.5f} of pixels within 1e-4"
 
 
 def test_multi_pass_parity_hyperbolic(ctx):
@@ -163,7 +182,8 @@ NO_CULL = NO_CULL_FP
 
 
 @pytest.mark.parametrize("name,full", [("eu_ball", False), ("eu_unit", False), ("hy_cells", False),
-                                       ("hy_cells", True), ("eu_example", False), ("hy_example", False)])
+                                       ("hy_cells", True), ("eu_example", False), ("hy_example", False),
+                                       ("sdf", False), ("sdf_1024", False)])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full):
     """The culled, compacted nearest-hit scan must give bit-identical canvas
     and seeds to the element-by-element scan."""
