@@ -243,6 +243,53 @@ def cache_leg(args):
     return out
 
 
+def sdf_leg(args):
+    """The sphere-traced scene at 1920x1080: 1024 SDF shapes, 256 spp, depth
+    16, Russian roulette from the 3rd hit, strict FP32: Mpaths/s of one frame
+    (one render_passes launch, inputs resident), and the FP64 reference image
+    at the same spp from independent seeds: RMSE between them next to the
+    FP64 image's own noise (its two halves)."""
+    from hypertrace_amd import scenes
+    from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render, program_options
+    from hypertrace_amd.types import X32_F32, X32_F64
+
+    wl = scenes.sdf_shapes()
+    W, H, SPP = wl.width, wl.height, args.sdf_spp
+    opts = program_options(rr_depth=3)
+    with Context(0, X32_F32) as ctx:
+        r = Render(ctx, wl.scene, fp_options=opts)
+        b = EntityBuffer(ctx, wl.scene)
+        c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        r.render_passes(b, c, 1)
+        c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.timer_begin()
+        r.render_passes(b, c, SPP)
+        gpu_ms = ctx.timer_end()
+        wall = time.perf_counter() - t0
+        img32 = gamma_image(c.load_image())
+    halves = []
+    t64 = time.perf_counter()
+    with Context(0, X32_F64) as c64:
+        r64 = Render(c64, wl.scene, fp_options=opts)
+        b64 = EntityBuffer(c64, wl.scene)
+        for k in range(2):
+            cc = Canvas(c64, (W, H), 1, wl.seed ^ ORACLE_SEED_XOR ^ (k + 1))
+            r64.render_passes(b64, cc, SPP // 2)
+            halves.append(cc.load_image())
+    t64 = time.perf_counter() - t64
+    ref = gamma_image(halves[0] + halves[1])
+    return {"workload": f"sdf_shapes 1920x1080, 1024 shapes (sphere:box:torus 1:1:1), {SPP} spp, depth 16, "
+                        "Russian roulette from the 3rd hit", "dtype": "f32",
+            "value": W * H * SPP / wall / 1e6, "unit": "Mpaths/s", "ms_per_frame": wall * 1e3,
+            "gpu_ms_per_frame": gpu_ms,
+            "fp64_reference": {"spp": 2 * (SPP // 2), "render_s": t64,
+                               "rmse_fp32_vs_fp64": rmse(img32, ref),
+                               "fp64_noise_rmse": rmse(gamma_image(halves[0]), gamma_image(halves[1])) / 2.0,
+                               "note": "independent seeds: the RMSE is MC noise of both images plus any FP32 bias"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +299,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="time only the chosen form")
     ap.add_argument("--no-cache", action="store_true", help="skip the 1080p radiance-cache leg")
     ap.add_argument("--cache-max-spp", type=int, default=1024)
+    ap.add_argument("--no-sdf", action="store_true", help="skip the 1080p SDF leg")
+    ap.add_argument("--sdf-spp", type=int, default=256)
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
@@ -379,6 +428,11 @@ def main():
             out["radiance_cache_1080p"] = cache_leg(args)
         except Exception as e:  # the headline stays valid if this leg fails
             out["radiance_cache_1080p"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_sdf:
+        try:
+            out["sdf_1080p"] = sdf_leg(args)
+        except Exception as e:
+            out["sdf_1080p"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, W, H, seeds0)
     print(json.dumps(out))

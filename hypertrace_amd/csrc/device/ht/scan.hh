@@ -196,14 +196,15 @@ struct CullRayEu {
         return {(double)r.start.x, (double)r.start.y, (double)r.start.z,
                 (double)r.direction.x, (double)r.direction.y, (double)r.direction.z, true};
     }
+    // Branch-free: every term is evaluated, so several elements' tests can
+    // be interleaved; the decision is the same boolean function.
     HD bool candidate(const Bound &b) const {
-        if (b.r < 0.f || !valid) return true;
         double sx = b.cx - px, sy = b.cy - py, sz = b.cz - pz;
         double r = b.r;
         double t = sx * dx + sy * dy + sz * dz;
-        if (t < -r) return false;  // wholly behind the start
         double qx = sx - t * dx, qy = sy - t * dy, qz = sz - t * dz;
-        return qx * qx + qy * qy + qz * qz <= r * r;
+        bool hit = !(t < -r) && qx * qx + qy * qy + qz * qz <= r * r;  // not wholly behind, and within r
+        return b.r < 0.f || !valid || hit;
     }
 };
 
@@ -235,41 +236,41 @@ struct CullRayHy {
         return c;
     }
     HD bool toward_tangent_point(double tx, double ty) const;
+    // Branch-free (the same boolean function as the sequence of tests it
+    // describes), so several elements' tests can be interleaved.
     HD bool candidate(const Bound &b) const {
-        if (b.r < 0.f || !valid) return true;
         double rx = (double)b.cx - px, ry = (double)b.cy - py;
         double ro = b.r, ri = b.ri;
         double delta = -rx * uy + ry * ux;  // distance from the vertical plane
         double d2p = delta * delta;
         double rp2 = ro * ro - d2p;  // squared radius of the plane's cut of the outer sphere
-        if (rp2 < 0.0) return false;
         double sig = rx * ux + ry * uy;  // along the plane, from p
         double sz = b.cz;
-        // the cut circle has a point ahead of p (horizontal coordinate >= 0);
-        // not applied to a horoball when the ray is near the geodesic through
-        // its tangent point T: there the element's local ray is near vertical,
-        // and the horosphere test (shape/hy/horosphere.cc:16-24) applies its
-        // EPS window before dividing by |d.xy|^2, so it accepts crossings
-        // behind the start.
-        if (sig < 0.0 && sig * sig > rp2 && !(b.horo > 0.f && toward_tangent_point(b.cx, b.cy))) return false;
         double ex = sig - gam;
         double d2 = ex * ex + sz * sz;  // squared distance between the circle centres
         double rg2 = rg * rg;
-        // disjoint: d2 > (rg + rp)^2  <=>  d2 - rg^2 - rp^2 > 2 rg rp
+        // 1. the plane misses the outer sphere
+        bool cull = rp2 < 0.0;
+        // 2. the cut circle lies wholly behind p (horizontal coordinate < 0);
+        //    not applied to a horoball when the ray is near the geodesic
+        //    through its tangent point T: there the element's local ray is
+        //    near vertical, and the horosphere test (shape/hy/horosphere.cc:16-24)
+        //    applies its EPS window before dividing by |d.xy|^2, so it accepts
+        //    crossings behind the start.
+        bool behind = sig < 0.0 && sig * sig > rp2;
+        if (behind && b.horo > 0.f) behind = !toward_tangent_point(b.cx, b.cy);
+        cull = cull || behind;
+        // 3. disjoint circles: d2 > (rg + rp)^2  <=>  d2 - rg^2 - rp^2 > 2 rg rp
         double o = d2 - rg2 - rp2;
-        if (o > 0.0 && o * o > 4.0 * rg2 * rp2) return false;
-        // outer cut circle strictly inside the geodesic circle: rp < rg, d2 < (rg - rp)^2
-        if (rg2 > rp2) {
-            double in = rg2 + rp2 - d2;
-            if (in > 0.0 && in * in > 4.0 * rg2 * rp2) return false;
-        }
-        // geodesic circle strictly inside the inner sphere's cut: rg < rpi, d2 < (rpi - rg)^2
+        cull = cull || (o > 0.0 && o * o > 4.0 * rg2 * rp2);
+        // 4. outer cut circle strictly inside the geodesic circle: rp < rg, d2 < (rg - rp)^2
+        double in1 = rg2 + rp2 - d2;
+        cull = cull || (rg2 > rp2 && in1 > 0.0 && in1 * in1 > 4.0 * rg2 * rp2);
+        // 5. geodesic circle strictly inside the inner sphere's cut: rg < rpi, d2 < (rpi - rg)^2
         double rpi2 = ri * ri - d2p;
-        if (rpi2 > rg2) {
-            double in = rg2 + rpi2 - d2;
-            if (in > 0.0 && in * in > 4.0 * rg2 * rpi2) return false;
-        }
-        return true;
+        double in2 = rg2 + rpi2 - d2;
+        cull = cull || (rpi2 > rg2 && in2 > 0.0 && in2 * in2 > 4.0 * rg2 * rpi2);
+        return b.r < 0.f || !valid || !cull;
     }
 };
 
@@ -435,10 +436,11 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             typename CullRayOf<G>::type cr = CullRayOf<G>::type::make(light->ray);
             wave_sync();
             uint qcount = 0;
-            for (usize i = 0; i < n; ++i) {
-                bool cand = cr.candidate(bounds[i]);
+            // Appends element i for the lanes where it is a candidate; runs
+            // the exact tests of a full batch of pairs as soon as there is one.
+            auto append = [&](usize i, bool cand) {
                 unsigned long long m = __ballot(cand);
-                if (m == 0) continue;
+                if (m == 0) return;
                 if (cand) ws->queue[qcount + (uint)__popcll(m & below)] = ((uint)i << 6) | lane;
                 qcount += (uint)__popcll(m);
                 if (qcount >= nact) {
@@ -452,7 +454,18 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                     qcount = rem;
                     wave_sync();
                 }
+            };
+            // Four independent cull tests per step (instruction-level parallelism).
+            usize e = 0;
+            for (; e + 4 <= n; e += 4) {
+                bool c0 = cr.candidate(bounds[e]), c1 = cr.candidate(bounds[e + 1]);
+                bool c2 = cr.candidate(bounds[e + 2]), c3 = cr.candidate(bounds[e + 3]);
+                append(e, c0);
+                append(e + 1, c1);
+                append(e + 2, c2);
+                append(e + 3, c3);
             }
+            for (; e < n; ++e) append(e, cr.candidate(bounds[e]));
             if (qcount > 0) {
                 wave_sync();
                 process(self, ws, arank, qcount);

@@ -976,14 +976,14 @@ struct CullRayEu {
         return {CntD(tofloat(p.x)), CntD(tofloat(p.y)), CntD(tofloat(p.z)),
                 CntD(tofloat(d.x)), CntD(tofloat(d.y)), CntD(tofloat(d.z))};
     }
+    // branch-free, as the kernel evaluates it: every term is computed
     bool candidate(const BoundF &b) const {
-        if (b.r < 0.f) return true;
         CntD sx = CntD(b.cx) - px, sy = CntD(b.cy) - py, sz = CntD(b.cz) - pz;
         CntD r = b.r;
         CntD t = sx * dx + sy * dy + sz * dz;
-        if (t.v < -r.v) return false;
         CntD qx = sx - t * dx, qy = sy - t * dy, qz = sz - t * dz;
-        return (qx * qx + qy * qy + qz * qz).v <= (r * r).v;
+        bool hit = !(t.v < -r.v) && (qx * qx + qy * qy + qz * qz).v <= (r * r).v;
+        return b.r < 0.f || hit;
     }
 };
 
@@ -1025,31 +1025,33 @@ struct CullRayHy {
         CntD den2 = h * h * v2 + tz * tz;
         return (den2 - n * n).v < (CntD(1e-2) * den2).v;
     }
+    // branch-free, as the kernel evaluates it: every term is computed; the
+    // tangent-point test only for a horoball whose cut lies behind p
     bool candidate(const BoundF &b) const {
-        if (b.r < 0.f || !valid) return true;
         CntD rx = CntD(b.cx) - px, ry = CntD(b.cy) - py;
         CntD ro = b.r, ri = b.ri;
         CntD delta = CntD(0.0) - rx * uy + ry * ux;
         CntD d2p = delta * delta;
         CntD rp2 = ro * ro - d2p;
-        if (rp2.v < 0.0) return false;
         CntD sig = rx * ux + ry * uy;
-        if (sig.v < 0.0 && (sig * sig).v > rp2.v && !(b.horo && toward_tangent_point(b.cx, b.cy))) return false;
         CntD ex = sig - gam;
         CntD d2 = ex * ex + CntD(b.cz) * CntD(b.cz);
         CntD rg2 = rg * rg;
+        bool cull = rp2.v < 0.0;
+        bool behind = sig.v < 0.0 && (sig * sig).v > rp2.v;
+        if (behind && b.horo) behind = !toward_tangent_point(b.cx, b.cy);
+        cull = cull || behind;
         CntD o = d2 - rg2 - rp2;
-        if (o.v > 0.0 && (o * o).v > (CntD(4.0) * rg2 * rp2).v) return false;
-        if (rg2.v > rp2.v) {
-            CntD in = rg2 + rp2 - d2;
-            if (in.v > 0.0 && (in * in).v > (CntD(4.0) * rg2 * rp2).v) return false;
-        }
+        CntD four_rr = CntD(4.0) * rg2 * rp2;
+        cull = cull || (o.v > 0.0 && (o * o).v > four_rr.v);
+        CntD in1 = rg2 + rp2 - d2;
+        CntD in1sq = in1 * in1;
+        cull = cull || (rg2.v > rp2.v && in1.v > 0.0 && in1sq.v > four_rr.v);
         CntD rpi2 = ri * ri - d2p;
-        if (rpi2.v > rg2.v) {
-            CntD in = rg2 + rpi2 - d2;
-            if (in.v > 0.0 && (in * in).v > (CntD(4.0) * rg2 * rpi2).v) return false;
-        }
-        return true;
+        CntD in2 = rg2 + rpi2 - d2;
+        CntD in2sq = in2 * in2, four_ri = CntD(4.0) * rg2 * rpi2;
+        cull = cull || (rpi2.v > rg2.v && in2.v > 0.0 && in2sq.v > four_ri.v);
+        return b.r < 0.f || !valid || !cull;
     }
 };
 
