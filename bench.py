@@ -290,6 +290,57 @@ def sdf_leg(args):
                                "note": "independent seeds: the RMSE is MC noise of both images plus any FP32 bias"}}
 
 
+def batch_leg(args):
+    """32 uneven jobs (half Euclidean, half hyperbolic; 128x128..1024x768
+    log-uniform, spp U{1..64}, depth U{4..12}, 8..256 primitives of the
+    diffuse / mirror / glass 1.5 mix) in one persistent launch per frame:
+    the autotuner's table (block size x tile size x grid), then frames of the
+    best candidate eager and as hipGraph replays (canvas clears, launch and
+    the post pass of every job), next to the same kernel run one job per
+    launch."""
+    from hypertrace_amd import scenes
+    from hypertrace_amd.batch import Batch, BatchProgram, autotune
+    from hypertrace_amd.processing import Context
+    from hypertrace_amd.types import X32_F32
+
+    specs = scenes.batch_jobs()
+    paths = sum(s.width * s.height * s.spp for s in specs)
+    with Context(0, X32_F32) as ctx:
+        tune = autotune(ctx, specs, reps=1)
+        best = tune["best"]
+        bprog = BatchProgram(ctx, best["block"])
+        batch = Batch(ctx, bprog, specs, best["tile"], with_rgba8=True)
+        grid = best["grid"]
+
+        def timed(fn, n=3):
+            fn()
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                fn()
+            ctx.sync()
+            return (time.perf_counter() - t0) / n
+
+        eager_s = timed(lambda: batch.render(grid, clear=True, post=True))
+        batch.capture(grid, clear=True, post=True)
+        graph_s = timed(batch.replay)
+        singles = [Batch(ctx, bprog, [s], best["tile"]) for s in specs]
+
+        def one_by_one():
+            for b in singles:
+                b.render(grid, clear=True)
+
+        single_s = timed(one_by_one, 2)
+    return {"workload": "32 jobs, half Euclidean / half hyperbolic, 128x128..1024x768, spp 1..64, depth 4..12, "
+                        "8..256 primitives, diffuse/mirror/glass 0.6/0.25/0.15",
+            "paths_per_frame": paths, "autotuned": best,
+            "graph_replay": {"ms_per_frame": graph_s * 1e3, "value": paths / graph_s / 1e6, "unit": "Mpaths/s"},
+            "eager": {"ms_per_frame": eager_s * 1e3, "value": paths / eager_s / 1e6, "unit": "Mpaths/s"},
+            "one_job_per_launch": {"ms_per_frame": single_s * 1e3, "value": paths / single_s / 1e6,
+                                   "unit": "Mpaths/s"},
+            "tuner_table": tune["table"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +351,7 @@ def main():
     ap.add_argument("--no-cache", action="store_true", help="skip the 1080p radiance-cache leg")
     ap.add_argument("--cache-max-spp", type=int, default=1024)
     ap.add_argument("--no-sdf", action="store_true", help="skip the 1080p SDF leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batched leg")
     ap.add_argument("--sdf-spp", type=int, default=256)
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
@@ -428,6 +480,11 @@ def main():
             out["radiance_cache_1080p"] = cache_leg(args)
         except Exception as e:  # the headline stays valid if this leg fails
             out["radiance_cache_1080p"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_batch:
+        try:
+            out["batch_32_jobs"] = batch_leg(args)
+        except Exception as e:
+            out["batch_32_jobs"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_sdf:
         try:
             out["sdf_1080p"] = sdf_leg(args)

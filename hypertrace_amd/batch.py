@@ -1,0 +1,167 @@
+"""Batched render of many jobs in one persistent launch (host side of
+`ht_batch_*` in include/hypertrace_hip.h), its hipGraph frame, and the
+autotuner that picks the block size and tile size of the batch program.
+
+A batch program is compiled for two scene types (Euclidean and hyperbolic
+jobs); block size (64..1024 threads) and tile size (pixels per work item)
+are compile-time, so each candidate of the tuner is its own program (hipRTC,
+cached on disk like every program). The tuner times whole frames of the
+caller's job set on the GPU and keeps the fastest candidate; the result is
+stored in a JSON file keyed by device and job-set signature.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import json
+import os
+import time
+from ctypes import Structure, c_uint32, c_void_p
+from typing import Dict, List, Optional, Sequence, Tuple
+
+from . import codegen, native
+from .processing import Buffer, Canvas, Context, EntityBuffer, ImageBuffer, Program, precompile
+from .scenes import BatchJobSpec, batch_euclidean_scene, batch_hyperbolic_scene
+from .types import Config
+
+import numpy as np
+
+
+class _Job(Structure):
+    _fields_ = [("type", c_uint32), ("w", c_uint32), ("h", c_uint32), ("spp", c_uint32), ("depth", c_uint32),
+                ("scene", c_void_p), ("canvas", c_void_p), ("seeds", c_void_p), ("rgba8", c_void_p)]
+
+
+def _lib():
+    return native.lib()
+
+
+def prototype_scenes():
+    """Scene instances of the two batch types (only their types matter)."""
+    rng = np.random.default_rng(0)
+    return batch_euclidean_scene(rng, 8), batch_hyperbolic_scene(rng, 9)
+
+
+def batch_root(config: Config, block: int, max_elements: int = 256) -> str:
+    s0, s1 = prototype_scenes()
+    return codegen.batch_source(s0, s1, config, block, max_elements)
+
+
+class BatchProgram:
+    def __init__(self, ctx: Context, block: int = 256, max_elements: int = 256, fp_options: Optional[str] = None):
+        self.block = block
+        self.program = Program(ctx, None, name="batch.hip", fp_options=fp_options,
+                               source=batch_root(ctx.config, block, max_elements))
+
+
+def precompile_batch(config: Config, block: int, fp_options: Optional[str] = None) -> bool:
+    return precompile(None, config, name="batch.hip", fp_options=fp_options, source=batch_root(config, block))
+
+
+class Batch:
+    """The device-side job set of one frame: buffers of every job and the
+    job table."""
+
+    def __init__(self, ctx: Context, bprog: BatchProgram, specs: Sequence[BatchJobSpec], tile: Optional[int] = None,
+                 with_rgba8: bool = False):
+        self.ctx = ctx
+        self.bprog = bprog
+        self.tile = tile if tile is not None else bprog.block
+        self.specs = list(specs)
+        self.scene_bufs = [EntityBuffer(ctx, s.scene) for s in self.specs]
+        self.canvases = [Canvas(ctx, (s.width, s.height), 1, s.seed) for s in self.specs]
+        self.rgba8 = [ImageBuffer(ctx, (s.width, s.height), 4, np.uint8) if with_rgba8 else None for s in self.specs]
+        arr = (_Job * len(self.specs))()
+        for k, (s, b, c, r) in enumerate(zip(self.specs, self.scene_bufs, self.canvases, self.rgba8)):
+            arr[k] = _Job(s.type, s.width, s.height, s.spp, s.depth, b.handle, c.image.handle, c.seeds.handle,
+                          r.handle if r is not None else None)
+        h = c_void_p()
+        native.check(_lib().ht_batch_create(bprog.program.handle, len(self.specs), ctypes.cast(arr, c_void_p),
+                                            self.tile, ctypes.byref(h)))
+        self.handle = h
+        self.tiles = _lib().ht_batch_tiles(h)
+        self.paths = sum(s.width * s.height * s.spp for s in self.specs)
+
+    def close(self):
+        if self.handle:
+            _lib().ht_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, grid: int, clear: bool = True, gamma: float = 1 / 2.2, post: bool = False) -> None:
+        native.check(_lib().ht_batch_render(self.handle, grid, int(clear), gamma, int(post)))
+        for c, s in zip(self.canvases, self.specs):
+            if clear:
+                c.passes = 0
+            c.add_pass(s.spp)
+
+    def capture(self, grid: int, clear: bool = True, gamma: float = 1 / 2.2, post: bool = False) -> None:
+        native.check(_lib().ht_batch_capture(self.handle, grid, int(clear), gamma, int(post)))
+        self._graph_clear = clear
+
+    def replay(self) -> None:
+        native.check(_lib().ht_batch_replay(self.handle))
+        for c, s in zip(self.canvases, self.specs):
+            if self._graph_clear:
+                c.passes = 0
+            c.add_pass(s.spp)
+
+
+def _signature(specs: Sequence[BatchJobSpec]) -> str:
+    h = hashlib.sha1()
+    for s in specs:
+        h.update(f"{s.type},{s.width},{s.height},{s.spp},{s.depth},{len(s.scene.object.items)};".encode())
+    return h.hexdigest()[:16]
+
+
+BLOCKS = (64, 128, 256, 512, 1024)
+TILE_MULTS = (1, 2, 4)
+CU_COUNT = 256
+
+
+def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mults=TILE_MULTS,
+             grid_waves_per_cu=(8, 16), reps: int = 2, cache_path: Optional[str] = None) -> Dict:
+    """Times one frame of `specs` for every (block, tile, grid) candidate and
+    returns the fastest with the whole table. grid = blocks such that each
+    CU gets the given number of waves."""
+    key = f"gfx950:{_signature(specs)}"
+    if cache_path and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            known = json.load(f)
+        if key in known:
+            return known[key]
+    table = []
+    for block in blocks:
+        bprog = BatchProgram(ctx, block)
+        for m in tile_mults:
+            tile = block * m
+            batch = Batch(ctx, bprog, specs, tile)
+            for wpc in grid_waves_per_cu:
+                grid = max(1, CU_COUNT * wpc * 64 // block)
+                batch.render(grid)  # warm
+                ctx.sync()
+                times = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    batch.render(grid)
+                    ctx.sync()
+                    times.append(time.perf_counter() - t0)
+                table.append({"block": block, "tile": tile, "grid": grid, "ms": 1e3 * min(times)})
+            batch.close()
+    best = min(table, key=lambda r: r["ms"])
+    result = {"best": best, "table": table, "paths": sum(s.width * s.height * s.spp for s in specs)}
+    if cache_path:
+        known = {}
+        if os.path.exists(cache_path):
+            with open(cache_path) as f:
+                known = json.load(f)
+        known[key] = result
+        os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+        with open(cache_path, "w") as f:
+            json.dump(known, f, indent=1)
+    return result

@@ -35,6 +35,7 @@ def build_library() -> str:
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     tmp = LIB + ".tmp"
     run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(PKG, "csrc", "runtime.hip"), os.path.join(PKG, "csrc", "cache.hip"),
+         os.path.join(PKG, "csrc", "batch.hip"),
          "-L/opt/rocm/lib", "-lhiprtc",
          "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
     os.replace(tmp, LIB)
@@ -72,12 +73,35 @@ def precompile_programs(verbose: bool = False) -> None:
                       f"{'cached' if hit else 'compiled'} in {time.time() - t:.1f}s")
 
 
+def precompile_batch_programs(verbose: bool = False) -> None:
+    """The batch programs of the autotuner's candidates, and the per-job
+    programs (each batch scene type with HOPS = 4..12) that render jobs one
+    at a time for comparison."""
+    from . import scenes
+    from .batch import BLOCKS, precompile_batch
+    from .objects import SceneImpl
+    from .processing import precompile
+    from .types import X32_F32
+
+    t = time.time()
+    for block in BLOCKS:
+        precompile_batch(X32_F32, block)
+    rng = __import__("numpy").random.default_rng(0)
+    protos = [scenes.batch_euclidean_scene(rng, 8), scenes.batch_hyperbolic_scene(rng, 9)]
+    for s in protos:
+        for hops in range(4, scenes.BATCH_MAX_HOPS + 1):
+            precompile(SceneImpl(s.geo, s.view, s.object, s.background, hops), X32_F32)
+    if verbose:
+        print(f"  batch and per-job programs in {time.time() - t:.1f}s")
+
+
 def build_all(verbose: bool = True) -> None:
     t = time.time()
     build_library()
     if verbose:
         print(f"built {LIB} in {time.time() - t:.1f}s")
     precompile_programs(verbose)
+    precompile_batch_programs(verbose)
 
 
 if __name__ == "__main__":

@@ -99,3 +99,29 @@ def root_source(scene: SceneImpl, cfg: Config) -> str:
     if cached:
         lines.append("HT_DEFINE_CACHED_KERNEL(Scene)")
     return "\n".join(lines) + "\n"
+
+
+def batch_source(scene0: SceneImpl, scene1: SceneImpl, cfg: Config, block: int = 256,
+                 max_elements: int = 256) -> str:
+    """Root source of a batch program: one persistent kernel (`render_batch`)
+    over jobs of two scene types; `block` threads per block (64..1024),
+    culled-scan bounds for up to `max_elements` top-level objects per scene
+    (larger scenes fall back to the plain scan)."""
+    if block % 64 or not 64 <= block <= 1024:
+        raise ValueError("block must be 64..1024 in steps of 64")
+    cap = 64
+    while cap < max_elements:
+        cap *= 2
+    lines = [
+        "// Generated batch program: two scene types, one persistent kernel.",
+        f"#define HT_BOUNDS_CAP {min(cap, 4096)}",
+        f"#define HT_MAX_WAVES {block // 64}",
+        f"#define HT_BATCH_BLOCK {block}",
+        "#include <ht/batch.hh>",
+        f"typedef {scene0.cpp()} Scene0;",
+        f"typedef {scene1.cpp()} Scene1;",
+    ]
+    lines += layout_asserts(scene0, cfg)
+    lines += layout_asserts(scene1, cfg)
+    lines.append("HT_DEFINE_BATCH_KERNEL(Scene0, Scene1)")
+    return "\n".join(dict.fromkeys(lines)) + "\n"

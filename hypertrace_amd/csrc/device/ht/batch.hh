@@ -1,0 +1,78 @@
+// Batched render of many jobs (scenes of up to two types, views, image
+// sizes, spp and depths) in one persistent launch.
+//
+// The frame's work is cut into tiles of `tile_px` pixels of one job (a
+// multiple of the block size),
+// numbered job after job (BatchJob::tile_begin). Each block takes the next
+// tile from a global counter, stages that job's scene bounds for the culled
+// scan (TopScan::prepare), and renders the tile's pixels, `spp` passes each
+// with in-thread path regeneration and at most `depth` hits per path; then it
+// takes the next tile. Blocks that finish early take more tiles, so uneven
+// jobs balance dynamically; the launch ends when the counter passes the last
+// tile. Per pixel the work is exactly `render_passes` of that job's scene with
+// HOPS = depth, so every job's canvas and seeds are bit-identical to
+// rendering it alone.
+#pragma once
+#include "render.hh"
+
+#ifndef HT_BATCH_BLOCK
+#define HT_BATCH_BLOCK 256
+#endif
+
+namespace ht {
+
+static_assert(HT_BATCH_BLOCK % 64 == 0 && HT_BATCH_BLOCK <= 64 * HT_MAX_WAVES,
+              "the scan's per-wave LDS scratch must cover every wave of a batch block");
+
+// One job, written by the host (64 bytes; device pointers as integers).
+struct BatchJob {
+    unsigned long long scene, canvas, seeds;
+    uint type, w, h, spp, depth, tile_begin, tiles, pad0, pad1, pad2;
+};
+static_assert(sizeof(BatchJob) == 64, "BatchJob layout");
+
+template <class S>
+HD void batch_tile(const BatchJob &job, uint tile, uint tile_px) {
+    const u8 *scene = reinterpret_cast<const u8 *>(job.scene);
+    TopScan<typename S::Object>::prepare(S::object(scene));
+    __syncthreads();
+    uint npx = job.w * job.h;
+    int hops = (int)job.depth < S::HOPS ? (int)job.depth : S::HOPS;
+    for (uint k = threadIdx.x; k < tile_px; k += HT_BATCH_BLOCK) {
+        uint idx = tile * tile_px + k;
+        if (idx < npx)
+            render_pixel_passes<S>(job.w, job.h, idx, scene, reinterpret_cast<float4 *>(job.canvas),
+                                   reinterpret_cast<uint *>(job.seeds), job.spp, hops);
+    }
+    __syncthreads();  // the bounds are restaged for the next tile
+}
+
+template <class S0, class S1>
+HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *__restrict__ counter,
+                          uint total_tiles, uint tile_px) {
+    __shared__ uint next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        __syncthreads();
+        uint t = next;
+        __syncthreads();
+        if (t >= total_tiles) return;
+        uint j = 0;
+        while (j + 1 < njobs && jobs[j + 1].tile_begin <= t) ++j;
+        const BatchJob job = jobs[j];
+        if (job.type == 0) {
+            batch_tile<S0>(job, t - job.tile_begin, tile_px);
+        } else {
+            batch_tile<S1>(job, t - job.tile_begin, tile_px);
+        }
+    }
+}
+
+}  // namespace ht
+
+#define HT_DEFINE_BATCH_KERNEL(S0, S1)                                                                   \
+    extern "C" __global__ void __launch_bounds__(HT_BATCH_BLOCK)                                         \
+        render_batch(const ht::BatchJob *__restrict__ jobs, unsigned njobs, unsigned *__restrict__ counter, \
+                     unsigned total_tiles, unsigned tile_px) {                                           \
+        ht::render_batch_impl<S0, S1>(jobs, njobs, counter, total_tiles, tile_px);                       \
+    }

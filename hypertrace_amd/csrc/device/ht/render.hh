@@ -108,15 +108,12 @@ HD void render_pass_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *_
     seeds[idx] = ctx.rng.state;
 }
 
+// `passes` samples of pixel idx with in-thread path regeneration, each path
+// at most `hops` hits (<= S::HOPS). The caller has prepared the scan.
 template <class S>
-HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
-                           uint *__restrict__ seeds, uint passes) {
+HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
+                            uint *__restrict__ seeds, uint passes, int hops) {
     typedef typename S::Geo G;
-    constexpr int H = S::HOPS;
-    TopScan<typename S::Object>::prepare(S::object(scene));
-    __syncthreads();
-    uint idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= w * h) return;
     uint x = idx % w, y = idx / w;
     PixelSetup px = pixel_setup(x, y, w, h);
     Context ctx;
@@ -126,7 +123,7 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 
     Light<G> light;
     color3 emission = c3(0.0f);
-    if constexpr (H <= 0) {
+    if (hops <= 0) {
         for (uint k = 0; k < passes; ++k) {
             start_path<S>(scene, &ctx, px, &light, &emission);
             acc.w += 1.0f;
@@ -144,7 +141,7 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
             }
             bool end = bounce<S>(scene, &ctx, &light, &emission, hop);
             ++hop;
-            if (hop >= H) end = true;
+            if (hop >= hops) end = true;
             if (end) {
                 acc.x += emission.x;
                 acc.y += emission.y;
@@ -157,6 +154,16 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
     }
     canvas[idx] = acc;
     seeds[idx] = ctx.rng.state;
+}
+
+template <class S>
+HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
+                           uint *__restrict__ seeds, uint passes) {
+    TopScan<typename S::Object>::prepare(S::object(scene));
+    __syncthreads();
+    uint idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= w * h) return;
+    render_pixel_passes<S>(w, h, idx, scene, canvas, seeds, passes, S::HOPS);
 }
 
 }  // namespace ht

@@ -36,7 +36,10 @@ namespace ht {
 
 static constexpr int BOUNDS_CAP = HT_BOUNDS_CAP;
 static constexpr int WAVE = 64;
-static constexpr int MAX_WAVES = 4;  // blocks of 256 threads
+#ifndef HT_MAX_WAVES
+#define HT_MAX_WAVES 4  // blocks of up to 256 threads
+#endif
+static constexpr int MAX_WAVES = HT_MAX_WAVES;
 
 // World-space sphere of an element: r >= the outer radius that contains the
 // primitive's image with margin (r < 0: no bound, always a candidate);

@@ -41,6 +41,7 @@ struct ht_prog {
     hipFunction_t render = nullptr;
     hipFunction_t render_passes = nullptr;
     hipFunction_t render_cached = nullptr;  // null unless the scene type supports the radiance cache
+    hipFunction_t render_batch = nullptr;   // batch programs (several scene types, one persistent kernel)
 };
 
 struct ht_buf {

@@ -436,16 +436,24 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
         delete p;
         return fail(HT_ERR_BACKEND, std::string("hipModuleLoadData: ") + hipGetErrorString(e));
     }
-    if (hipModuleGetFunction(&p->render, p->module, "render") != hipSuccess ||
-        hipModuleGetFunction(&p->render_passes, p->module, "render_passes") != hipSuccess) {
+    // Scene programs have render/render_passes (and render_cached where the
+    // radiance cache applies); batch programs have render_batch.
+    auto find = [&](const char *name) -> hipFunction_t {
+        hipFunction_t f = nullptr;
+        if (hipModuleGetFunction(&f, p->module, name) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return f;
+    };
+    p->render = find("render");
+    p->render_passes = find("render_passes");
+    p->render_cached = find("render_cached");
+    p->render_batch = find("render_batch");
+    if (!(p->render && p->render_passes) && !p->render_batch) {
         (void)hipModuleUnload(p->module);
         delete p;
-        return fail(HT_ERR_BACKEND, "program lacks the render/render_passes kernels");
-    }
-    // Present only for scene types the radiance cache supports.
-    if (hipModuleGetFunction(&p->render_cached, p->module, "render_cached") != hipSuccess) {
-        p->render_cached = nullptr;
-        (void)hipGetLastError();
+        return fail(HT_ERR_BACKEND, "program lacks the render/render_passes (or render_batch) kernels");
     }
     ctx_retain(ctx);
     *out = p;
@@ -550,6 +558,7 @@ int check_image(const char *what, uint32_t w, uint32_t h, const ht_buf *b, size_
 int launch_render(ht_prog *prog, hipFunction_t fn, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas,
                   ht_buf *seeds, uint32_t passes, bool with_passes) {
     if (!prog || !scene) return fail(HT_ERR_OTHER, "render: invalid argument");
+    if (!fn) return fail(HT_ERR_OTHER, "render: the program has no per-scene render kernels (a batch program)");
     if (w == 0 || h == 0) return HT_OK;
     if ((uint64_t)w * (uint64_t)h > 0xFFFFFFFFull) return fail(HT_ERR_OTHER, "render: image too large");
     int rc;

@@ -51,6 +51,13 @@ SIGNATURES = {
     "ht_render_cached": (c_int, [c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32,
                                  c_uint32, c_uint32]),
     "ht_cache_train": (c_int, [c_void_p, c_uint32, c_uint32, c_float]),
+    "ht_batch_create": (c_int, [c_void_p, c_uint32, c_void_p, c_uint32, POINTER(c_void_p)]),
+    "ht_batch_destroy": (None, [c_void_p]),
+    "ht_batch_tiles": (c_uint32, [c_void_p]),
+    "ht_batch_block": (c_uint32, [c_void_p]),
+    "ht_batch_render": (c_int, [c_void_p, c_uint32, c_int, c_float, c_int]),
+    "ht_batch_capture": (c_int, [c_void_p, c_uint32, c_int, c_float, c_int]),
+    "ht_batch_replay": (c_int, [c_void_p]),
     "ht_seed_init": (c_int, [c_void_p, c_uint32, c_uint32, c_int, c_uint64]),
     "ht_seed_fill": (c_int, [POINTER(c_uint32), c_size_t, c_int, c_uint64]),
     "ht_seed_hash": (c_uint32, [c_uint64, c_uint32]),

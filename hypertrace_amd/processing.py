@@ -181,10 +181,12 @@ def seed_fill(n: int, mode: int = 0, seed: int = REFERENCE_SEED) -> np.ndarray:
 class Program:
     """A compiled render program for one scene type."""
 
-    def __init__(self, ctx: Context, scene: SceneImpl, name: str = "render.hip", fp_options: Optional[str] = None):
+    def __init__(self, ctx: Context, scene: Optional[SceneImpl], name: str = "render.hip",
+                 fp_options: Optional[str] = None, source: Optional[str] = None):
+        """`source` replaces the generated root source (batch programs)."""
         self.ctx = ctx
         self.fp_options = fp_options
-        self.source = codegen.root_source(scene, ctx.config)
+        self.source = source if source is not None else codegen.root_source(scene, ctx.config)
         hdrs = codegen.device_headers()
         names = native.cstr_array(list(hdrs.keys()))
         contents = native.cstr_array(list(hdrs.values()))
@@ -227,10 +229,11 @@ FAST_FP = "-fno-hip-fp32-correctly-rounded-divide-sqrt"
 NO_CULL_FP = STRICT_FP + " -DHT_NO_CULL=1"  # strict, element-by-element scan
 
 
-def precompile(scene: SceneImpl, config: Config, name: str = "render.hip", fp_options: Optional[str] = None) -> bool:
-    """Compile a scene's program into the code-object cache (no GPU needed).
-    Returns True if it was already cached."""
-    source = codegen.root_source(scene, config)
+def precompile(scene: Optional[SceneImpl], config: Config, name: str = "render.hip",
+               fp_options: Optional[str] = None, source: Optional[str] = None) -> bool:
+    """Compile a scene's program (or the given root source) into the
+    code-object cache (no GPU needed). Returns True if it was already cached."""
+    source = source if source is not None else codegen.root_source(scene, config)
     hdrs = codegen.device_headers()
     lib = native.lib()
     native.check(lib.ht_prog_precompile(name.encode(), source.encode(), len(hdrs),

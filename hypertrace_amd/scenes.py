@@ -274,3 +274,85 @@ def reference_hyperbolic_example(width: int = 800, height: int = 600, spp: int =
     view = Mapped(PointView(1.0), cam)
     scene = SceneImpl(HY, view, ObjectVec(objs), ConstBg(_unpack_color(0xEEEEEE)), 3)
     return Workload("reference_hyperbolic_example", scene, width, height, spp, 0, 0xDEADBEEFDEADBEEF)
+
+
+# ------------------------------------------------------------ batched jobs
+
+def _mix(rng):
+    """One-hot mixture: diffuse (albedo U[0.3, 0.9]^3) / mirror / glass
+    (IOR 1.5) with probabilities 0.6 / 0.25 / 0.15."""
+    from .objects import Refractive
+
+    u = rng.uniform()
+    onehot = (1.0, 0.0, 0.0) if u < 0.6 else (0.0, 1.0, 0.0) if u < 0.85 else (0.0, 0.0, 1.0)
+    albedo = rng.uniform(0.3, 0.9, 3)
+    return Mixture([(Colored(Lambertian(), albedo), onehot[0]), (Specular(), onehot[1]), (Refractive(1.5), onehot[2])])
+
+
+BATCH_MAX_HOPS = 12
+
+
+def batch_euclidean_scene(rng, prims: int, hops: int = BATCH_MAX_HOPS) -> SceneImpl:
+    """Batch type 0: balls (radius U[0.2, 0.8], centres U[-5, 5]^3) under a
+    gradient sky, pinhole at (0, 0, 14), 60 degree vertical FOV."""
+    objs = [Mapped(Covered(Ball(rng.uniform(0.2, 0.8)), _mix(rng)), Shift3(rng.uniform(-5.0, 5.0, 3)))
+            for _ in range(prims)]
+    view = Mapped(PointView(math.tan(math.radians(30.0))), Homogenous3((0.0, 0.0, 14.0), (1, 0, 0, 0)))
+    bg = GradBg((0.0, 1.0, 0.0), [(1.0, 1.0, 1.0), (0.08, 0.08, 0.1)], 1.0)
+    return SceneImpl(EU, view, ObjectVec(objs), bg, hops)
+
+
+def batch_hyperbolic_scene(rng, prims: int, hops: int = BATCH_MAX_HOPS) -> SceneImpl:
+    """Batch type 1: cells of 1 horosphere + 2 planes as in the cell scene,
+    each primitive's map a word of 1..6 generators (FP64, reprojected) times
+    its local map; constant sky; camera at zshift(ln 2)."""
+    from .maps import compose_word
+
+    choice = [Plane(), Horosphere()]
+    objs = []
+    letters, dh = [], 0.25
+    for i in range(prims):
+        if i % 3 == 0:
+            letters = [int(rng.integers(0, 4)) for _ in range(int(rng.integers(1, 7)))]
+            dh = rng.uniform(0.15, 0.35)
+            kind, local = 1, _translate(0.0).chain(_scale(dh)).chain(_inversion())
+        else:
+            kind = 0
+            local = _translate(complex(*rng.uniform(-0.5, 0.5, 2))).chain(_scale(rng.uniform(0.1, 0.3)))
+        m = compose_word(CELL_GENERATORS, letters, local)
+        objs.append(Mapped(Covered(ShapeChoice(choice, kind), _mix(rng)), m))
+    view = Mapped(PointView(1.0), hy_zshift(math.log(2.0)))
+    return SceneImpl(HY, view, ObjectVec(objs), ConstBg((1.0, 1.0, 1.0)), hops)
+
+
+@dataclass
+class BatchJobSpec:
+    type: int  # 0 Euclidean, 1 hyperbolic
+    scene: SceneImpl
+    width: int
+    height: int
+    spp: int
+    depth: int
+    seed: int
+
+
+def batch_jobs(n: int = 32, min_size=(128, 128), max_size=(1024, 768), spp_range=(1, 64), depth_range=(4, 12),
+               prim_range=(8, 256), depths=None) -> List[BatchJobSpec]:
+    """The batched workload: n independent jobs, half Euclidean, half
+    hyperbolic (alternating), width and height log-uniform between min_size
+    and max_size, spp U{spp_range}, depth U{depth_range} (or drawn from
+    `depths`), U{prim_range} primitives with the diffuse / mirror / glass mix.
+    Seeds: per-pixel hash seeding with MASTER_SEED ^ 3 ^ (job << 8)."""
+    rng = _rng(3)
+    jobs = []
+    for j in range(n):
+        w = int(round(math.exp(rng.uniform(math.log(min_size[0]), math.log(max_size[0])))))
+        h = int(round(math.exp(rng.uniform(math.log(min_size[1]), math.log(max_size[1])))))
+        spp = int(rng.integers(spp_range[0], spp_range[1] + 1))
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1)) if depths is None else int(
+            depths[int(rng.integers(0, len(depths)))])
+        prims = int(rng.integers(prim_range[0], prim_range[1] + 1))
+        t = j % 2
+        scene = batch_euclidean_scene(rng, prims) if t == 0 else batch_hyperbolic_scene(rng, prims)
+        jobs.append(BatchJobSpec(t, scene, w, h, spp, depth, MASTER_SEED ^ 3 ^ (j << 8)))
+    return jobs

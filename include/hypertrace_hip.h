@@ -171,6 +171,38 @@ int ht_render_cached(ht_prog *prog, ht_cache *cache, uint32_t w, uint32_t h, ht_
  * pass (sample i of step k is record (k * batch + i) mod count). Asynchronous. */
 int ht_cache_train(ht_cache *cache, uint32_t steps, uint32_t batch, float lr);
 
+/* ---- Batched render: many jobs (scenes of the batch program's two types,
+ * views, sizes, spp, depths) in one persistent launch. No reference
+ * counterpart: the reference builds one program per scene type and renders
+ * one scene per launch (processing/src/render.rs:14-118).
+ *
+ * A batch program is built with ht_prog_build_ex from a root source that
+ * instantiates HT_DEFINE_BATCH_KERNEL(Scene0, Scene1) (codegen.batch_source);
+ * its block size is the kernel's launch bound; `tile` (pixels per work item,
+ * a multiple of the block size) is chosen at ht_batch_create. Per pixel the work equals ht_render_passes of the
+ * job's scene with at most `depth` hits per path, so canvases and seeds are
+ * bit-identical to rendering each job alone. */
+typedef struct ht_batch ht_batch;
+typedef struct ht_batch_job {
+    uint32_t type;             /* 0 or 1: Scene0 or Scene1 of the batch program */
+    uint32_t w, h, spp, depth; /* image, passes per frame, hits per path (<= the type's HOPS) */
+    ht_buf *scene, *canvas, *seeds;
+    ht_buf *rgba8;             /* optional: post-pass output of each frame */
+} ht_batch_job;
+int ht_batch_create(ht_prog *batch_prog, uint32_t njobs, const ht_batch_job *jobs, uint32_t tile, ht_batch **out);
+void ht_batch_destroy(ht_batch *batch);
+uint32_t ht_batch_tiles(const ht_batch *batch);
+uint32_t ht_batch_block(const ht_batch *batch);
+/* One frame on the context's stream (asynchronous): clear = zero every
+ * job's canvas first (a frame of `spp` passes; else the passes accumulate),
+ * then the persistent launch over `grid` blocks; post = ht_post of each job
+ * with an rgba8 buffer (needs clear). */
+int ht_batch_render(ht_batch *batch, uint32_t grid, int clear, float gamma, int post);
+/* The same frame captured into a hipGraph (clears by kernels, no memset
+ * nodes), and its replay. */
+int ht_batch_capture(ht_batch *batch, uint32_t grid, int clear, float gamma, int post);
+int ht_batch_replay(ht_batch *batch);
+
 /* Per-pixel seed buffer. Replaces Canvas::new's seed fill
  * (processing/src/canvas.rs:73-84).
  * mode 0: rand 0.8 SmallRng::seed_from_u64(seed).fill(&mut [u32]) on 64-bit

@@ -1,0 +1,89 @@
+"""The batched persistent render on the GPU: every job's canvas and seeds
+bit-identical to rendering that job alone (its own program with HOPS =
+depth), for several block and tile sizes; hipGraph replays identical to
+eager frames; argument checks."""
+import numpy as np
+import pytest
+
+from hypertrace_amd import scenes
+from hypertrace_amd.batch import Batch, BatchProgram
+from hypertrace_amd.native import HtError
+from hypertrace_amd.objects import SceneImpl
+from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
+from hypertrace_amd.types import X32_F32
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(n=8, min_size=(24, 16), max_size=(96, 72), spp_range=(1, 4), prim_range=(8, 64), depths=(4, 7, 12))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0, X32_F32)
+    yield c
+    c.close()
+
+
+def alone(ctx, spec, frames=1):
+    """The job rendered by itself: its scene with HOPS = depth, render_passes."""
+    s = spec.scene
+    scene = SceneImpl(s.geo, s.view, s.object, s.background, spec.depth)
+    r = Render(ctx, scene)
+    buf = EntityBuffer(ctx, scene)
+    c = Canvas(ctx, (spec.width, spec.height), 1, spec.seed)
+    for _ in range(frames):
+        r.render_passes(buf, c, spec.spp)
+    return c.load_image(), c.load_seeds()
+
+
+@pytest.mark.parametrize("block,mult", [(256, 1), (64, 4), (1024, 1)])
+def test_batch_jobs_identical_to_single_renders(ctx, block, mult):
+    specs = scenes.batch_jobs(**SMALL)
+    print(f"case block {block} tile {block * mult}", flush=True)
+    bprog = BatchProgram(ctx, block)
+    batch = Batch(ctx, bprog, specs, block * mult)
+    batch.render(grid=7, clear=False)
+    ctx.sync()
+    for k, spec in enumerate(specs):
+        img, seeds = alone(ctx, spec)
+        got = batch.canvases[k]
+        np.testing.assert_array_equal(got.load_image().view(np.uint32), img.view(np.uint32), err_msg=f"job {k}")
+        np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
+
+
+def test_graph_replays_match_eager_frames(ctx):
+    specs = scenes.batch_jobs(**SMALL)
+    bprog = BatchProgram(ctx, 256)
+    eager = Batch(ctx, bprog, specs, with_rgba8=True)
+    graph = Batch(ctx, bprog, specs, with_rgba8=True)
+    graph.capture(grid=16, clear=True, post=True)
+    other = scenes.euclidean_spheres(32, 24, 1)
+    r_other = Render(ctx, other.scene)
+    b_other = EntityBuffer(ctx, other.scene)
+    c_other = Canvas(ctx, (32, 24), 1, 5)
+    for frame in range(3):
+        eager.render(grid=16, clear=True, post=True)
+        graph.replay()
+        r_other.render_passes(b_other, c_other, 2)  # unrelated eager work between replays
+        ctx.sync()
+        for k in range(len(specs)):
+            a, b = eager.canvases[k].load_image(), graph.canvases[k].load_image()
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=f"frame {frame} job {k}")
+            np.testing.assert_array_equal(eager.rgba8[k].load(), graph.rgba8[k].load())
+            assert np.all(a[..., 3] == specs[k].spp)  # each frame starts from a cleared canvas
+
+
+def test_batch_argument_checks(ctx):
+    specs = scenes.batch_jobs(**SMALL)[:2]
+    bprog = BatchProgram(ctx, 256)
+    with pytest.raises(HtError, match="tile"):
+        Batch(ctx, bprog, specs, 128)  # not a multiple of the block
+    wl = scenes.euclidean_spheres(16, 16, 1)
+    scene_prog = Render(ctx, wl.scene).program
+
+    class NotBatch:
+        program = scene_prog
+        block = 256
+
+    with pytest.raises(HtError, match="not a batch program"):
+        Batch(ctx, NotBatch(), specs)
