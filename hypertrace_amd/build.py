@@ -63,8 +63,10 @@ def precompile_programs(verbose: bool = False) -> None:
     from .types import X32_F32, X32_F64
 
     rr = program_options(rr_depth=3)
+    tree = program_options() + " -DHT_CULL_TREE=1"
     for scene in standard_scenes():
         extra = [(X32_F32, rr), (X32_F64, rr)] if scene.geo == "Eu" else []
+        extra.append((X32_F32, tree))
         for cfg, fp in [(X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP), (X32_F32, NO_CULL_FP)] + extra:
             t = time.time()
             hit = precompile(scene, cfg, fp_options=fp)

@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from hypertrace_amd import scenes
-from hypertrace_amd.processing import (NO_CULL_FP, Canvas, Context, EntityBuffer, Extractor, GammaFilter, ImageBuffer, Packer,
+from hypertrace_amd.processing import (NO_CULL_FP, STRICT_FP, Canvas, Context, EntityBuffer, Extractor, GammaFilter, ImageBuffer, Packer,
                                        Pipeline, Render, program_options, seed_fill)
 from hypertrace_amd.types import X32_F32, X32_F64
 
@@ -181,17 +181,21 @@ def test_pipeline_fused_post_equals_separate(ctx):
 NO_CULL = NO_CULL_FP
 
 
-@pytest.mark.parametrize("name,full", [("eu_ball", False), ("eu_unit", False), ("hy_cells", False),
-                                       ("hy_cells", True), ("eu_example", False), ("hy_example", False),
-                                       ("sdf", False), ("sdf_1024", False)])
-def test_culled_scan_identical_to_plain_scan(ctx, name, full):
-    """The culled, compacted nearest-hit scan must give bit-identical canvas
-    and seeds to the element-by-element scan."""
+@pytest.mark.parametrize("name,full,tree", [("eu_ball", False, False), ("eu_unit", False, False),
+                                            ("hy_cells", False, False), ("hy_cells", True, False),
+                                            ("eu_example", False, False), ("hy_example", False, False),
+                                            ("sdf", False, False), ("sdf_1024", False, False),
+                                            ("hy_cells", False, True), ("sdf_1024", False, True)])
+def test_culled_scan_identical_to_plain_scan(ctx, name, full, tree):
+    """The culled, compacted nearest-hit scan (flat, or walking the sphere
+    tree with -DHT_CULL_TREE) must give bit-identical canvas and seeds to
+    the element-by-element scan."""
     wl = scenes.hyperbolic_cells() if full else WORKLOADS[name]()
-    print(f"case {name} full={full}", flush=True)
+    print(f"case {name} full={full} tree={tree}", flush=True)
     buf = EntityBuffer(ctx, wl.scene)
     out = []
-    for fp in (None, NO_CULL):
+    culled = (STRICT_FP + " -DHT_CULL_TREE=1") if tree else None
+    for fp in (culled, NO_CULL):
         r = Render(ctx, wl.scene, fp_options=fp)
         c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
         r.render_passes(buf, c, 4)
