@@ -206,7 +206,9 @@ def cache_leg(args):
     with Context(0, X32_F32) as ctx:
         buf = EntityBuffer(ctx, wl.scene)
         compose_on_device(ctx, buf, wl.scene, wl.word_maps)
-        chosen = Render(ctx, wl.scene)
+        from hypertrace_amd.tuning import tune_render
+        tuning = tune_render(ctx, wl.scene, W, H, 8, wl.seed_mode, wl.seed)
+        chosen = Render(ctx, wl.scene, fp_options=tuning["best"]["options"])
         plain = None if args.no_plain else Render(ctx, wl.scene, fp_options=NO_CULL)
         base = time_to_rmse(ctx, wl, buf, chosen, plain, args, reference)
         cache = RadianceCache(ctx, W * H, 65536, 16384)
@@ -231,6 +233,7 @@ def cache_leg(args):
     out = {"workload": "hyperbolic_cells 1920x1080, 256 primitives, 8 hops", "target": args.target_rmse,
            "oracle_spp": reference[3], "oracle_noise_rmse": reference[1], "oracle_render_s": reference[2],
            "bit_exact_forms": {k: base.get(k) for k in ("reached", "spp", "chosen_s", "plain_s", "speedup")},
+           "chosen_program": tuning["best"],
            "cached": {"reached": reached is not None, "spp": reached if reached is not None else spp,
                       "seconds": t, "ms_per_pass": t * 1e3 / max(spp, 1), "curve": curve[::4] + curve[-1:],
                       "queries_last_pass": queries, "training_records_last_pass": records,
@@ -401,7 +404,13 @@ def main():
             ctx.sync()
             return time.perf_counter() - t0, gpu_ms
 
-        chosen = Render(ctx, wl.scene)  # default program: strict IEEE FP32, culled scan
+        # chosen program: strict IEEE FP32, culled scan, with the unroll /
+        # waves-per-SIMD setting the tuner picks for this frame (all settings
+        # render bit-identical images)
+        from hypertrace_amd.tuning import tune_render
+        tuning = tune_render(ctx, wl.scene, W, H, SPP, wl.seed_mode, wl.seed)
+        extra["render_tuning"] = tuning
+        chosen = Render(ctx, wl.scene, fp_options=tuning["best"]["options"])
         wall, gpu_ms = time_chosen(chosen, args.steps, args.warmup)
         ms_per_step = wall * 1e3 / args.steps
         kernel_ms = gpu_ms / args.steps

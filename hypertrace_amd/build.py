@@ -62,11 +62,17 @@ def precompile_programs(verbose: bool = False) -> None:
     from .processing import FAST_FP, NO_CULL_FP, precompile, program_options
     from .types import X32_F32, X32_F64
 
+    from .tuning import CANDIDATES, options
+
     rr = program_options(rr_depth=3)
     tree = program_options() + " -DHT_CULL_TREE=1"
-    for scene in standard_scenes():
+    scenes_ = standard_scenes()
+    for k, scene in enumerate(scenes_):
         extra = [(X32_F32, rr), (X32_F64, rr)] if scene.geo == "Eu" else []
-        extra.append((X32_F32, tree))
+        if k in (2, 5):  # the cell scene and the 1024-shape scene: the sphere-tree option
+            extra.append((X32_F32, tree))
+        if k == 2:  # the cell scene: the render tuner's candidates
+            extra += [(X32_F32, options(u, w)) for u, w in CANDIDATES]
         for cfg, fp in [(X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP), (X32_F32, NO_CULL_FP)] + extra:
             t = time.time()
             hit = precompile(scene, cfg, fp_options=fp)

@@ -169,14 +169,20 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 }  // namespace ht
 
 // Kernel entry points for the scene type `S`, with C linkage so the runtime
-// finds them by name in the compiled code object.
+// finds them by name in the compiled code object. HT_WAVES_PER_EU: the
+// minimum resident waves per SIMD the register allocation must allow (4:
+// at most 128 VGPRs; measured best for 640x480 frames; 5 pays a small
+// spill for more occupancy on larger frames; see tuning.py).
+#ifndef HT_WAVES_PER_EU
+#define HT_WAVES_PER_EU 4
+#endif
 #define HT_DEFINE_RENDER_KERNELS(S)                                                                   \
-    extern "C" __global__ void __launch_bounds__(256) render(                                         \
+    extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU) render(                        \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas, \
         unsigned *__restrict__ seeds) {                                                               \
         ht::render_pass_impl<S>(w, h, scene, canvas, seeds);                                          \
     }                                                                                                 \
-    extern "C" __global__ void __launch_bounds__(256) render_passes(                                  \
+    extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU) render_passes(                 \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas, \
         unsigned *__restrict__ seeds, unsigned passes) {                                              \
         ht::render_passes_impl<S>(w, h, scene, canvas, seeds, passes);                                \

@@ -35,6 +35,10 @@
 namespace ht {
 
 static constexpr int BOUNDS_CAP = HT_BOUNDS_CAP;
+#ifndef HT_CULL_UNROLL
+#define HT_CULL_UNROLL 4
+#endif
+static constexpr int CULL_UNROLL = HT_CULL_UNROLL;
 static constexpr int WAVE = 64;
 #ifndef HT_MAX_WAVES
 #define HT_MAX_WAVES 4  // blocks of up to 256 threads
@@ -45,11 +49,17 @@ static constexpr int MAX_WAVES = HT_MAX_WAVES;
 // primitive's image with margin (r < 0: no bound, always a candidate);
 // ri <= an inner radius that the image contains with margin (used only for
 // "the geodesic stays inside the sphere", where shrinking is the
-// conservative direction).
+// conservative direction); horo = 1: a horoball, tangent to the boundary
+// below its centre (2: a tree node whose subtree holds one). FP32, 32 B:
+// measured on MI355X, FP64 storage with the squares precomputed (64 B, four
+// LDS reads per element) is slower (39.4 vs 38.0 ms per cell frame).
 struct Bound {
     float cx, cy, cz, r;
-    float ri, horo, pad1, pad2;  // horo = 1: a horoball, tangent to the boundary below its centre
+    float ri, horo, pad1, pad2;
 };
+HD double b_r2(const Bound &b) { return (double)b.r * (double)b.r; }
+HD double b_ri2(const Bound &b) { return (double)b.ri * (double)b.ri; }
+HD double b_cz2(const Bound &b) { return (double)b.cz * (double)b.cz; }
 
 // Relative cull margin for an element whose exact test runs through a map
 // with squared Frobenius norm n2: generous against the FP32 rounding of the
@@ -119,20 +129,37 @@ template <class G, class TI, class M> struct Cullable<Mapped<G, TI, M>> {
 };
 
 // ------------------------------------------------------------ world bounds
-HD Bound no_bound() { return {0.f, 0.f, 0.f, -1.f, -1.f, 0.f, 0.f, 0.f}; }
-
-// Outer radius inflated, inner radius shrunk, in FP64, each rounded
-// outwards (resp. inwards) to FP32 together with the centre's rounding.
+template <class B> struct MakeBound;
+template <> struct MakeBound<Bound> {
+    static HD Bound none() { return {0.f, 0.f, 0.f, -1.f, -1.f, 0.f, 0.f, 0.f}; }
+    // Outer radius inflated, inner radius shrunk, in FP64, each rounded
+    // outwards (resp. inwards) to FP32 together with the centre's rounding.
+    static HD Bound make(double cx, double cy, double cz, double r, double margin) {
+        float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+        double err = fabs_((double)fx - cx) + fabs_((double)fy - cy) + fabs_((double)fz - cz);
+        double ro = r * (1.0 + margin) + err + 1e-30;
+        float fo = (float)ro;
+        if ((double)fo < ro) fo = __uint_as_float(__float_as_uint(fo) + 1u);  // next float up (fo > 0)
+        double rin = r * (1.0 - margin) - err;
+        float fi = rin > 0.0 ? (float)rin : 0.0f;
+        if ((double)fi > rin && fi > 0.0f) fi = __uint_as_float(__float_as_uint(fi) - 1u);  // next float down
+        return {fx, fy, fz, fo, fi, 0.f, 0.f, 0.f};
+    }
+    static HD Bound node(float4 q, bool horo) { return {q.x, q.y, q.z, q.w, 0.f, horo ? 2.f : 0.f, 0.f, 0.f}; }
+};
+HD Bound no_bound() { return MakeBound<Bound>::none(); }
 HD Bound make_bound(double cx, double cy, double cz, double r, double margin) {
-    float fx = (float)cx, fy = (float)cy, fz = (float)cz;
-    double err = fabs_((double)fx - cx) + fabs_((double)fy - cy) + fabs_((double)fz - cz);
-    double ro = r * (1.0 + margin) + err + 1e-30;
-    float fo = (float)ro;
-    if ((double)fo < ro) fo = __uint_as_float(__float_as_uint(fo) + 1u);  // next float up (fo > 0)
-    double rin = r * (1.0 - margin) - err;
-    float fi = rin > 0.0 ? (float)rin : 0.0f;
-    if ((double)fi > rin && fi > 0.0f) fi = __uint_as_float(__float_as_uint(fi) - 1u);  // next float down
-    return {fx, fy, fz, fo, fi, 0.f, 0.f, 0.f};
+    return MakeBound<Bound>::make(cx, cy, cz, r, margin);
+}
+
+// The element sphere as FP32 (x, y, z, r), rounded outwards (tree leaves).
+HD float4 leaf_sphere(const Bound &b) {
+    float fx = (float)b.cx, fy = (float)b.cy, fz = (float)b.cz;
+    double err = fabs_((double)fx - (double)b.cx) + fabs_((double)fy - (double)b.cy) + fabs_((double)fz - (double)b.cz);
+    double ro = (double)b.r + err;
+    float fr = (float)ro;
+    if ((double)fr < ro) fr = __uint_as_float(__float_as_uint(fr) + 1u);
+    return make_float4(fx, fy, fz, fr);
 }
 
 template <class M> struct WorldBound;
@@ -202,12 +229,12 @@ struct CullRayEu {
     // Branch-free: every term is evaluated, so several elements' tests can
     // be interleaved; the decision is the same boolean function.
     HD bool candidate(const Bound &b) const {
-        double sx = b.cx - px, sy = b.cy - py, sz = b.cz - pz;
+        double sx = (double)b.cx - px, sy = (double)b.cy - py, sz = (double)b.cz - pz;
         double r = b.r;
         double t = sx * dx + sy * dy + sz * dz;
         double qx = sx - t * dx, qy = sy - t * dy, qz = sz - t * dz;
-        bool hit = !(t < -r) && qx * qx + qy * qy + qz * qz <= r * r;  // not wholly behind, and within r
-        return b.r < 0.f || !valid || hit;
+        bool hit = !(t < -r) && qx * qx + qy * qy + qz * qz <= b_r2(b);  // not wholly behind, and within r
+        return b.r < 0 || !valid || hit;
     }
 };
 
@@ -216,7 +243,7 @@ struct CullRayEu {
 // centre at p.xy + gam u (gam = p.z d.z / |d.xy|), radius p.z / |d.xy|; the
 // part ahead of p is the part with horizontal coordinate >= 0 along u.
 struct CullRayHy {
-    double px, py, pz, dx, dy, dz, ux, uy, gam, rg;
+    double px, py, pz, dx, dy, dz, ux, uy, gam, rg, rg2, four_rg2;
     bool valid;
     static HD CullRayHy make(const Ray<Hy> &r) {
         CullRayHy c;
@@ -236,6 +263,8 @@ struct CullRayHy {
         c.uy = dy / dxy;
         c.gam = pz * dz / dxy;
         c.rg = pz * dn / dxy;
+        c.rg2 = c.rg * c.rg;
+        c.four_rg2 = 4.0 * c.rg2;
         return c;
     }
     HD bool toward_tangent_point(double tx, double ty) const;
@@ -243,15 +272,12 @@ struct CullRayHy {
     // describes), so several elements' tests can be interleaved.
     HD bool candidate(const Bound &b) const {
         double rx = (double)b.cx - px, ry = (double)b.cy - py;
-        double ro = b.r, ri = b.ri;
         double delta = -rx * uy + ry * ux;  // distance from the vertical plane
         double d2p = delta * delta;
-        double rp2 = ro * ro - d2p;  // squared radius of the plane's cut of the outer sphere
+        double rp2 = b_r2(b) - d2p;  // squared radius of the plane's cut of the outer sphere
         double sig = rx * ux + ry * uy;  // along the plane, from p
-        double sz = b.cz;
         double ex = sig - gam;
-        double d2 = ex * ex + sz * sz;  // squared distance between the circle centres
-        double rg2 = rg * rg;
+        double d2 = ex * ex + b_cz2(b);  // squared distance between the circle centres
         // 1. the plane misses the outer sphere
         bool cull = rp2 < 0.0;
         // 2. the cut circle lies wholly behind p (horizontal coordinate < 0);
@@ -262,19 +288,20 @@ struct CullRayHy {
         //    crossings behind the start.
         bool behind = sig < 0.0 && sig * sig > rp2;
         // horo = 2 marks a tree node whose subtree holds a horoball: no "behind" cull
-        if (behind && b.horo > 0.f) behind = b.horo < 1.5f && !toward_tangent_point(b.cx, b.cy);
+        if (behind && b.horo > 0) behind = b.horo < 1.5 && !toward_tangent_point(b.cx, b.cy);
         cull = cull || behind;
         // 3. disjoint circles: d2 > (rg + rp)^2  <=>  d2 - rg^2 - rp^2 > 2 rg rp
         double o = d2 - rg2 - rp2;
-        cull = cull || (o > 0.0 && o * o > 4.0 * rg2 * rp2);
+        double f_rp2 = four_rg2 * rp2;
+        cull = cull || (o > 0.0 && o * o > f_rp2);
         // 4. outer cut circle strictly inside the geodesic circle: rp < rg, d2 < (rg - rp)^2
         double in1 = rg2 + rp2 - d2;
-        cull = cull || (rg2 > rp2 && in1 > 0.0 && in1 * in1 > 4.0 * rg2 * rp2);
+        cull = cull || (rg2 > rp2 && in1 > 0.0 && in1 * in1 > f_rp2);
         // 5. geodesic circle strictly inside the inner sphere's cut: rg < rpi, d2 < (rpi - rg)^2
-        double rpi2 = ri * ri - d2p;
+        double rpi2 = b_ri2(b) - d2p;
         double in2 = rg2 + rpi2 - d2;
-        cull = cull || (rpi2 > rg2 && in2 > 0.0 && in2 * in2 > 4.0 * rg2 * rpi2);
-        return b.r < 0.f || !valid || !cull;
+        cull = cull || (rpi2 > rg2 && in2 > 0.0 && in2 * in2 > four_rg2 * rpi2);
+        return b.r < 0 || !valid || !cull;
     }
 };
 
@@ -475,8 +502,8 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         }
         __syncthreads();
         for (uint i = threadIdx.x; i < n; i += blockDim.x) {
-            if (b[i].r < 0.f) continue;
-            float c[3] = {b[i].cx, b[i].cy, b[i].cz};
+            if (b[i].r < 0) continue;
+            float c[3] = {(float)b[i].cx, (float)b[i].cy, (float)b[i].cz};
             for (int a = 0; a < 3; ++a) {
                 atomicMin(&t->lo[a], float_key(c[a]));
                 atomicMax(&t->hi[a], float_key(c[a]));
@@ -490,8 +517,8 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             unsigned long long k = ~0ull;
             if (i < n) {
                 uint code = 0xFFFFFFFFu;
-                if (b[i].r >= 0.f) {
-                    float c[3] = {b[i].cx, b[i].cy, b[i].cz};
+                if (b[i].r >= 0) {
+                    float c[3] = {(float)b[i].cx, (float)b[i].cy, (float)b[i].cz};
                     code = 0;
                     for (int a = 0; a < 3; ++a) {
                         float lo = key_float(t->lo[a]), hi = key_float(t->hi[a]);
@@ -540,8 +567,8 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                             h[c] = false;
                         } else {
                             const Bound &bb = b[e];
-                            s[c] = make_float4(bb.cx, bb.cy, bb.cz, bb.r < 0.f ? -1.f : bb.r);
-                            h[c] = bb.horo > 0.f;
+                            s[c] = bb.r < 0 ? make_float4(0.f, 0.f, 0.f, -1.f) : leaf_sphere(bb);
+                            h[c] = bb.horo > 0;
                         }
                     } else {
                         s[c] = t->node[ch];
@@ -661,10 +688,9 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                                 bb.r = -2.f;
                             }
                         } else {
-                            float4 q = t->node[node];
-                            bb = {q.x, q.y, q.z, q.w, 0.f, t->horo[node] ? 2.f : 0.f, 0.f, 0.f};
+                            bb = MakeBound<Bound>::node(t->node[node], t->horo[node] != 0);
                         }
-                        bool pass = bb.r != -2.f && cr.candidate(bb);
+                        bool pass = bb.r != -2 && cr.candidate(bb);
                         if (node >= P) {
                             cand = pass;
                         }
@@ -678,15 +704,15 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                     append(elem, cand);
                 }
             } else {
-                // Four independent cull tests per step (instruction-level parallelism).
+                // CULL_UNROLL independent cull tests per step (instruction-level
+                // parallelism, traded against registers and so occupancy).
                 usize e = 0;
-                for (; e + 4 <= n; e += 4) {
-                    bool c0 = cr.candidate(bounds[e]), c1 = cr.candidate(bounds[e + 1]);
-                    bool c2 = cr.candidate(bounds[e + 2]), c3 = cr.candidate(bounds[e + 3]);
-                    append(e, c0);
-                    append(e + 1, c1);
-                    append(e + 2, c2);
-                    append(e + 3, c3);
+                for (; e + CULL_UNROLL <= n; e += CULL_UNROLL) {
+                    bool c[CULL_UNROLL];
+#pragma unroll
+                    for (int u = 0; u < CULL_UNROLL; ++u) c[u] = cr.candidate(bounds[e + u]);
+#pragma unroll
+                    for (int u = 0; u < CULL_UNROLL; ++u) append(e + u, c[u]);
                 }
                 for (; e < n; ++e) append(e, cr.candidate(bounds[e]));
             }

@@ -346,3 +346,40 @@ class Pipeline:
 
     def clean(self) -> None:
         self.canvas.clean()
+
+    def save_image(self, path: str, fused: bool = True) -> np.ndarray:
+        """Post passes, then the RGB of the RGBA8 image to a binary PPM (P6)
+        file at `path` (for visual regression); returns the RGBA8 image."""
+        rgba = self.get_image(fused=fused)
+        write_ppm(path, rgba)
+        return rgba
+
+
+def write_ppm(path: str, rgba8: np.ndarray) -> None:
+    """Binary PPM (P6, maxval 255) of an [h, w, 4] uint8 image, rows top
+    to bottom as the canvas stores them (pixel x + w * y)."""
+    img = np.ascontiguousarray(np.asarray(rgba8, np.uint8)[..., :3])
+    h, w = img.shape[:2]
+    with open(path, "wb") as f:
+        f.write(f"P6\n{w} {h}\n255\n".encode())
+        f.write(img.tobytes())
+
+
+def read_ppm(path: str) -> np.ndarray:
+    """[h, w, 3] uint8 image of a binary PPM written by write_ppm."""
+    with open(path, "rb") as f:
+        data = f.read()
+    parts, pos = [], 0
+    while len(parts) < 4:
+        while data[pos:pos + 1].isspace():
+            pos += 1
+        end = pos
+        while not data[end:end + 1].isspace():
+            end += 1
+        parts.append(data[pos:end])
+        pos = end
+    pos += 1
+    if parts[0] != b"P6" or int(parts[3]) != 255:
+        raise ValueError("not a binary 8-bit PPM")
+    w, h = int(parts[1]), int(parts[2])
+    return np.frombuffer(data[pos:pos + 3 * w * h], np.uint8).reshape(h, w, 3)

@@ -995,7 +995,7 @@ struct CullRayEu {
 // the behind-the-start case (the horosphere test accepts crossings behind
 // the start for near-vertical local rays, shape/hy/horosphere.cc:16-24).
 struct CullRayHy {
-    CntD px, py, pz, dx, dy, dz, ux, uy, gam, rg;
+    CntD px, py, pz, dx, dy, dz, ux, uy, gam, rg, rg2, four_rg2;
     bool valid = false;
     template <class R> static CullRayHy make(const V4<R> &p, const V4<R> &d) {
         CullRayHy c;
@@ -1015,6 +1015,8 @@ struct CullRayHy {
         c.uy = y / dxy;
         c.gam = pz * z / dxy;
         c.rg = pz * dn / dxy;
+        c.rg2 = c.rg * c.rg;
+        c.four_rg2 = CntD(4.0) * c.rg2;
         return c;
     }
     bool toward_tangent_point(double tx, double ty) const {
@@ -1036,20 +1038,19 @@ struct CullRayHy {
         CntD sig = rx * ux + ry * uy;
         CntD ex = sig - gam;
         CntD d2 = ex * ex + CntD(b.cz) * CntD(b.cz);
-        CntD rg2 = rg * rg;
         bool cull = rp2.v < 0.0;
         bool behind = sig.v < 0.0 && (sig * sig).v > rp2.v;
         if (behind && b.horo) behind = !toward_tangent_point(b.cx, b.cy);
         cull = cull || behind;
         CntD o = d2 - rg2 - rp2;
-        CntD four_rr = CntD(4.0) * rg2 * rp2;
+        CntD four_rr = four_rg2 * rp2;
         cull = cull || (o.v > 0.0 && (o * o).v > four_rr.v);
         CntD in1 = rg2 + rp2 - d2;
         CntD in1sq = in1 * in1;
         cull = cull || (rg2.v > rp2.v && in1.v > 0.0 && in1sq.v > four_rr.v);
         CntD rpi2 = ri * ri - d2p;
         CntD in2 = rg2 + rpi2 - d2;
-        CntD in2sq = in2 * in2, four_ri = CntD(4.0) * rg2 * rpi2;
+        CntD in2sq = in2 * in2, four_ri = four_rg2 * rpi2;
         cull = cull || (rpi2.v > rg2.v && in2.v > 0.0 && in2sq.v > four_ri.v);
         return b.r < 0.f || !valid || !cull;
     }

@@ -185,7 +185,8 @@ NO_CULL = NO_CULL_FP
                                             ("hy_cells", False, False), ("hy_cells", True, False),
                                             ("eu_example", False, False), ("hy_example", False, False),
                                             ("sdf", False, False), ("sdf_1024", False, False),
-                                            ("hy_cells", False, True), ("sdf_1024", False, True)])
+                                            ("hy_cells", False, True), ("sdf_1024", False, True),
+                                            ("hy_cells", True, "unroll2_waves5")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, tree):
     """The culled, compacted nearest-hit scan (flat, or walking the sphere
     tree with -DHT_CULL_TREE) must give bit-identical canvas and seeds to
@@ -194,7 +195,10 @@ def test_culled_scan_identical_to_plain_scan(ctx, name, full, tree):
     print(f"case {name} full={full} tree={tree}", flush=True)
     buf = EntityBuffer(ctx, wl.scene)
     out = []
-    culled = (STRICT_FP + " -DHT_CULL_TREE=1") if tree else None
+    if tree == "unroll2_waves5":  # a tuner setting that spills: same images
+        culled = STRICT_FP + " -DHT_CULL_UNROLL=2 -DHT_WAVES_PER_EU=5"
+    else:
+        culled = (STRICT_FP + " -DHT_CULL_TREE=1") if tree else None
     for fp in (culled, NO_CULL):
         r = Render(ctx, wl.scene, fp_options=fp)
         c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)

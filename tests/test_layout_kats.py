@@ -90,3 +90,4 @@ def test_tiled_horosphere_entity_matches_the_ocl_layout():
     t = TiledHorosphere([m] * 3, 1.0, 0.02, m, "hexagonal")
     assert t.static_size(X32_F32) == 144
     assert len(t.to_bytes(X32_F32)) == 144
+
