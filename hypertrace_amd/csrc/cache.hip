@@ -84,7 +84,7 @@ constexpr int DEL_ROWS = 5 * HID + 16;
 
 constexpr float ADAM_B1 = 0.9f, ADAM_B2 = 0.99f, ADAM_EPS = 1e-8f;
 constexpr float LOSS_EPS = 0.01f;  // relative L2: (p - y)^2 / (p^2 + eps)
-constexpr int WGRAD_CHUNK = 512;   // samples per wgrad partial
+constexpr int WGRAD_CHUNK = 128;   // samples per wgrad partial (16 384 samples: 128 blocks, 512 waves)
 
 // kappa <-> feature (see the header comment); for K = 64 and K = 16 (t = 0 only).
 __host__ __device__ inline int kappa_of(int f) {

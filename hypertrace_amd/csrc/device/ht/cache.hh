@@ -232,7 +232,7 @@ HD void render_cached_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 }  // namespace ht
 
 #define HT_DEFINE_CACHED_KERNEL(S)                                                                             \
-    extern "C" __global__ void __launch_bounds__(256) render_cached(                                           \
+    extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU) render_cached(                          \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas,          \
         unsigned *__restrict__ seeds, unsigned pass, unsigned *__restrict__ counters,                          \
         unsigned *__restrict__ q_pixel, float *__restrict__ q_feat, float *__restrict__ q_thr, unsigned qcap,  \
