@@ -359,6 +359,9 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES",
+                    help="use this cull unroll / waves-per-SIMD setting instead of tuning (keeps a profiled "
+                         "trace to one program)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "
                          "the known-size copy that scales FETCH_SIZE / WRITE_SIZE counter readings")
@@ -407,8 +410,13 @@ def main():
         # chosen program: strict IEEE FP32, culled scan, with the unroll /
         # waves-per-SIMD setting the tuner picks for this frame (all settings
         # render bit-identical images)
-        from hypertrace_amd.tuning import tune_render
-        tuning = tune_render(ctx, wl.scene, W, H, SPP, wl.seed_mode, wl.seed)
+        from hypertrace_amd.tuning import options as knob_options, tune_render
+        if args.render_knobs:
+            unroll, waves = (int(v) for v in args.render_knobs.split(","))
+            tuning = {"best": {"unroll": unroll, "waves_per_eu": waves, "options": knob_options(unroll, waves)},
+                      "table": [], "fixed": True}
+        else:
+            tuning = tune_render(ctx, wl.scene, W, H, SPP, wl.seed_mode, wl.seed)
         extra["render_tuning"] = tuning
         chosen = Render(ctx, wl.scene, fp_options=tuning["best"]["options"])
         wall, gpu_ms = time_chosen(chosen, args.steps, args.warmup)

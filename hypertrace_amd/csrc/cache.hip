@@ -50,6 +50,7 @@ constexpr int IN = 144;      // encoded inputs
 constexpr int IN_PAD = 160;  // 5 k-steps of 32
 constexpr int HID = 64;
 constexpr int OUT = 3;
+static_assert(IN == NF * 2 * NBANDS, "encoding width");
 
 // FP32 master parameters: W0 [64][144], b0 [64], W1..W4 [64][64], b1..b4, W5 [3][64], b5 [3].
 constexpr int P_W0 = 0, P_B0 = P_W0 + HID * IN;
@@ -76,10 +77,8 @@ __host__ __device__ constexpr int i_b(int l) { return l == 5 ? I_B5 : I_BH + (l 
 // Activation / gradient buffers for training (feature-major, n columns):
 // act: A0 [160][n] (encoded input), A1..A5 [64][n] (layer inputs); del: D0..D4 [64][n], D5 [16][n].
 __host__ __device__ constexpr size_t act_row(int l) { return l == 0 ? 0 : (size_t)IN_PAD + (size_t)(l - 1) * HID; }
-__host__ __device__ constexpr int act_rows(int l) { return l == 0 ? IN_PAD : HID; }
 constexpr int ACT_ROWS = IN_PAD + 5 * HID;
 __host__ __device__ constexpr size_t del_row(int l) { return (size_t)l * HID; }
-__host__ __device__ constexpr int del_rows(int l) { return l == 5 ? 16 : HID; }
 constexpr int DEL_ROWS = 5 * HID + 16;
 
 constexpr float ADAM_B1 = 0.9f, ADAM_B2 = 0.99f, ADAM_EPS = 1e-8f;
@@ -335,7 +334,6 @@ __global__ void __launch_bounds__(256) k_mlp_train(const __bf16 *__restrict__ im
 struct WTile {
     int l, tn, tk;
 };
-__host__ __device__ constexpr int wtiles(int l) { return l == 0 ? 4 * 10 : l == 5 ? 1 * 4 : 4 * 4; }
 constexpr int NWTILES = 40 + 4 * 16 + 4;
 __device__ inline WTile wtile(int i) {
     if (i < 40) return {0, i / 10, i % 10};
@@ -345,15 +343,18 @@ __device__ inline WTile wtile(int i) {
     return {5, 0, i};
 }
 
-// Block x = chunk of WGRAD_CHUNK samples; its waves loop over the tiles.
-// partial[chunk][NPARAM]: every weight and bias written once per chunk.
-__global__ void __launch_bounds__(256) k_mlp_wgrad(const __bf16 *__restrict__ act, const __bf16 *__restrict__ del,
-                                                   uint32_t n, float *__restrict__ partial) {
+// Block x = chunk of WGRAD_CHUNK samples; its WGRAD_WAVES waves loop over
+// the tiles. partial[chunk][NPARAM]: every weight and bias written once per
+// chunk.
+constexpr int WGRAD_WAVES = 8;
+__global__ void __launch_bounds__(64 * WGRAD_WAVES) k_mlp_wgrad(const __bf16 *__restrict__ act,
+                                                                const __bf16 *__restrict__ del, uint32_t n,
+                                                                float *__restrict__ partial) {
     int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     uint32_t chunk = blockIdx.x;
     uint32_t s_begin = chunk * WGRAD_CHUNK, s_end = min(s_begin + WGRAD_CHUNK, n);
     float *out = partial + (size_t)chunk * NPARAM;
-    for (int ti = wave; ti < NWTILES; ti += 4) {
+    for (int ti = wave; ti < NWTILES; ti += WGRAD_WAVES) {
         WTile wt = wtile(ti);
         int l = wt.l;
         const __bf16 *dp = del + (del_row(l) + 16 * wt.tn + c) * (size_t)n;
@@ -415,15 +416,29 @@ __global__ void k_mlp_pack(const float *__restrict__ params, __bf16 *__restrict_
     if (i < NPARAM) pack_param(image, i, params[i]);
 }
 
-// Sums the chunk partials in order, then (if apply) one Adam step; writes
-// the gradient, the parameter and its BF16 image entries.
-__global__ void k_mlp_adam(float *__restrict__ params, float *__restrict__ m1, float *__restrict__ m2,
-                           const float *__restrict__ partial, uint32_t nchunks, float lr, float c1, float c2,
-                           int apply, float *__restrict__ grad, __bf16 *__restrict__ image) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= NPARAM) return;
-    float gsum = 0.0f;
-    for (uint32_t k = 0; k < nchunks; ++k) gsum += partial[(size_t)k * NPARAM + i];
+// Sums the chunk partials, then (if apply) one Adam step; writes the
+// gradient, the parameter and its BF16 image entries. A block holds
+// ADAM_PARAMS parameters x ADAM_SUBS sub-sums: sub-sum s adds chunks
+// s, s + ADAM_SUBS, ... in order, and the sub-sums are added in order of s,
+// so the result is deterministic. Lanes of one sub read consecutive
+// parameters (coalesced rows of the partial matrix).
+constexpr int ADAM_SUBS = 8, ADAM_PARAMS = 256 / ADAM_SUBS;
+__global__ void __launch_bounds__(256) k_mlp_adam(float *__restrict__ params, float *__restrict__ m1,
+                                                  float *__restrict__ m2, const float *__restrict__ partial,
+                                                  uint32_t nchunks, float lr, float c1, float c2, int apply,
+                                                  float *__restrict__ grad, __bf16 *__restrict__ image) {
+    __shared__ float sub_sum[ADAM_SUBS][ADAM_PARAMS];
+    int col = threadIdx.x % ADAM_PARAMS, sub = threadIdx.x / ADAM_PARAMS;
+    int i = blockIdx.x * ADAM_PARAMS + col;
+    float part = 0.0f;
+    if (i < NPARAM)
+        for (uint32_t k = sub; k < nchunks; k += ADAM_SUBS) part += partial[(size_t)k * NPARAM + i];
+    sub_sum[sub][col] = part;
+    __syncthreads();
+    if (sub != 0 || i >= NPARAM) return;
+    float gsum = sub_sum[0][col];
+#pragma unroll
+    for (int s = 1; s < ADAM_SUBS; ++s) gsum += sub_sum[s][col];
     grad[i] = gsum;
     if (!apply) return;
     float a = ADAM_B1 * m1[i] + (1.0f - ADAM_B1) * gsum;
@@ -491,7 +506,7 @@ int train_batch(ht_cache *c, const uint32_t *count_ptr, uint32_t count_const, ui
                        offset, n, feat, target, c->act, c->del, c->loss);
     HIP_TRY(hipGetLastError());
     uint32_t nchunks = (n + WGRAD_CHUNK - 1) / WGRAD_CHUNK;
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3(nchunks), dim3(256), 0, st, c->act, c->del, n, c->partial);
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3(nchunks), dim3(64 * WGRAD_WAVES), 0, st, c->act, c->del, n, c->partial);
     HIP_TRY(hipGetLastError());
     float c1 = 1.0f, c2 = 1.0f;
     if (apply) {
@@ -499,8 +514,8 @@ int train_batch(ht_cache *c, const uint32_t *count_ptr, uint32_t count_const, ui
         c1 = (float)(1.0 / (1.0 - std::pow((double)ADAM_B1, (double)c->step)));
         c2 = (float)(1.0 / (1.0 - std::pow((double)ADAM_B2, (double)c->step)));
     }
-    hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, st, c->params, c->m1, c->m2, c->partial,
-                       nchunks, lr, c1, c2, apply, c->grad, c->image);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((NPARAM + ADAM_PARAMS - 1) / ADAM_PARAMS), dim3(256), 0, st, c->params,
+                       c->m1, c->m2, c->partial, nchunks, lr, c1, c2, apply, c->grad, c->image);
     HIP_TRY(hipGetLastError());
     return HT_OK;
 }

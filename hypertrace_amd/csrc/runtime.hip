@@ -97,7 +97,7 @@ void write_file_atomic(const std::string &dir, const std::string &path, const st
 // ------------------------------------------------------------------ post kernels
 
 // image = canvas / (float)passes, processing/src/canvas.rs:126-135.
-__global__ void __launch_bounds__(256) k_extract(uint32_t n, float inv_unused, uint32_t passes,
+__global__ void __launch_bounds__(256) k_extract(uint32_t n, uint32_t passes,
                                                  const float4 *__restrict__ canvas, float4 *__restrict__ image) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -602,7 +602,7 @@ extern "C" int ht_extract(ht_ctx *ctx, uint32_t w, uint32_t h, uint32_t passes, 
     if (!n) return HT_OK;
     HIP_TRY(hipSetDevice(ctx->device));
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_extract, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, 0.0f, passes,
+    hipLaunchKernelGGL(k_extract, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, passes,
                        (const float4 *)canvas->ptr, (float4 *)image->ptr);
     HIP_TRY(hipGetLastError());
     return HT_OK;
