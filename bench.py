@@ -260,7 +260,10 @@ def sdf_leg(args):
     W, H, SPP = wl.width, wl.height, args.sdf_spp
     opts = program_options(rr_depth=3)
     with Context(0, X32_F32) as ctx:
-        r = Render(ctx, wl.scene, fp_options=opts)
+        # the render tuner over 4-spp frames (all settings render the same images)
+        from hypertrace_amd.tuning import tune_render
+        tuning = tune_render(ctx, wl.scene, W, H, 4, wl.seed_mode, wl.seed, base=opts)
+        r = Render(ctx, wl.scene, fp_options=tuning["best"]["options"])
         b = EntityBuffer(ctx, wl.scene)
         c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
         r.render_passes(b, c, 1)
@@ -286,7 +289,7 @@ def sdf_leg(args):
     return {"workload": f"sdf_shapes 1920x1080, 1024 shapes (sphere:box:torus 1:1:1), {SPP} spp, depth 16, "
                         "Russian roulette from the 3rd hit", "dtype": "f32",
             "value": W * H * SPP / wall / 1e6, "unit": "Mpaths/s", "ms_per_frame": wall * 1e3,
-            "gpu_ms_per_frame": gpu_ms,
+            "gpu_ms_per_frame": gpu_ms, "render_tuning": tuning,
             "fp64_reference": {"spp": 2 * (SPP // 2), "render_s": t64,
                                "rmse_fp32_vs_fp64": rmse(img32, ref),
                                "fp64_noise_rmse": rmse(gamma_image(halves[0]), gamma_image(halves[1])) / 2.0,
@@ -359,9 +362,9 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
-    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES",
-                    help="use this cull unroll / waves-per-SIMD setting instead of tuning (keeps a profiled "
-                         "trace to one program)")
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb]",
+                    help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
+                         "instead of tuning (keeps a profiled trace to one program)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "
                          "the known-size copy that scales FETCH_SIZE / WRITE_SIZE counter readings")
@@ -412,8 +415,10 @@ def main():
         # render bit-identical images)
         from hypertrace_amd.tuning import options as knob_options, tune_render
         if args.render_knobs:
-            unroll, waves = (int(v) for v in args.render_knobs.split(","))
-            tuning = {"best": {"unroll": unroll, "waves_per_eu": waves, "options": knob_options(unroll, waves)},
+            knobs = args.render_knobs.split(",")
+            unroll, waves, sb = int(knobs[0]), int(knobs[1]), knobs[2:] == ["sb"]
+            tuning = {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb,
+                               "options": knob_options(unroll, waves, scalar_bounds=sb)},
                       "table": [], "fixed": True}
         else:
             tuning = tune_render(ctx, wl.scene, W, H, SPP, wl.seed_mode, wl.seed)

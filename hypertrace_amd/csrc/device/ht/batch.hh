@@ -13,6 +13,9 @@
 // HOPS = depth, so every job's canvas and seeds are bit-identical to
 // rendering it alone.
 #pragma once
+#ifdef HT_SCALAR_BOUNDS
+#error "batch programs scan a different scene per job; HT_SCALAR_BOUNDS keeps one bounds array per program"
+#endif
 #include "render.hh"
 
 #ifndef HT_BATCH_BLOCK

@@ -176,7 +176,19 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 #ifndef HT_WAVES_PER_EU
 #define HT_WAVES_PER_EU 4
 #endif
+// HT_SCALAR_BOUNDS programs also export bounds_fill(scene), which the
+// runtime launches before each render / render_passes / render_cached
+// launch of the program.
+#ifdef HT_SCALAR_BOUNDS
+#define HT_DEFINE_BOUNDS_KERNEL(S)                                                                          \
+    extern "C" __global__ void __launch_bounds__(256) bounds_fill(const unsigned char *__restrict__ scene) { \
+        ht::TopScan<typename S::Object>::fill_scalar(S::object(scene));                                     \
+    }
+#else
+#define HT_DEFINE_BOUNDS_KERNEL(S)
+#endif
 #define HT_DEFINE_RENDER_KERNELS(S)                                                                   \
+    HT_DEFINE_BOUNDS_KERNEL(S)                                                                        \
     extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU) render(                        \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas, \
         unsigned *__restrict__ seeds) {                                                               \

@@ -57,9 +57,70 @@ struct Bound {
     float cx, cy, cz, r;
     float ri, horo, pad1, pad2;
 };
+HD double b_cx(const Bound &b) { return (double)b.cx; }
+HD double b_cy(const Bound &b) { return (double)b.cy; }
+HD double b_cz(const Bound &b) { return (double)b.cz; }
+HD double b_r(const Bound &b) { return (double)b.r; }
 HD double b_r2(const Bound &b) { return (double)b.r * (double)b.r; }
 HD double b_ri2(const Bound &b) { return (double)b.ri * (double)b.ri; }
 HD double b_cz2(const Bound &b) { return (double)b.cz * (double)b.cz; }
+HD float b_horo(const Bound &b) { return b.horo; }
+
+// The same bound widened to FP64 with the squares precomputed (the values
+// the accessors above compute), 64 B: the layout of the scalar-load scan
+// (program option HT_SCALAR_BOUNDS), where a kernel fills one array of them
+// in global memory per launch and the scan reads element e's bound at a
+// wave-uniform index, so the loads go through the scalar cache into SGPRs
+// once per wave instead of as LDS reads and conversions in every lane.
+struct BoundS {
+    double cx, cy, r2, ri2, cz2, cz, r;
+    float horo, pad;
+};
+static_assert(sizeof(BoundS) == 64, "BoundS layout");
+HD double b_cx(const BoundS &b) { return b.cx; }
+HD double b_cy(const BoundS &b) { return b.cy; }
+HD double b_cz(const BoundS &b) { return b.cz; }
+HD double b_r(const BoundS &b) { return b.r; }
+HD double b_r2(const BoundS &b) { return b.r2; }
+HD double b_ri2(const BoundS &b) { return b.ri2; }
+HD double b_cz2(const BoundS &b) { return b.cz2; }
+HD float b_horo(const BoundS &b) { return b.horo; }
+HD BoundS widen(const Bound &b) {
+    BoundS w;
+    w.cx = b_cx(b);
+    w.cy = b_cy(b);
+    w.r2 = b_r2(b);
+    w.ri2 = b_ri2(b);
+    w.cz2 = b_cz2(b);
+    w.cz = b_cz(b);
+    w.r = b_r(b);
+    w.horo = b.horo;
+    w.pad = 0.0f;
+    return w;
+}
+#ifdef HT_SCALAR_BOUNDS
+// The top-level vector's bounds for the scalar-load scan; written by the
+// program's bounds_fill kernel before each render launch (launches of one
+// program on one stream are ordered, so one array per loaded program).
+__device__ BoundS ht_scalar_bounds[HT_BOUNDS_CAP];
+// The scan reads the array through a constant-address-space pointer (the
+// same memory; no kernel that reads it writes it), which lets wave-uniform
+// loads go through the scalar cache into SGPRs.
+typedef __attribute__((address_space(4))) const BoundS ConstBoundS;
+HD BoundS load_bound(const ConstBoundS *p) {
+    BoundS b;
+    b.cx = p->cx;
+    b.cy = p->cy;
+    b.r2 = p->r2;
+    b.ri2 = p->ri2;
+    b.cz2 = p->cz2;
+    b.cz = p->cz;
+    b.r = p->r;
+    b.horo = p->horo;
+    b.pad = 0.0f;
+    return b;
+}
+#endif
 
 // Relative cull margin for an element whose exact test runs through a map
 // with squared Frobenius norm n2: generous against the FP32 rounding of the
@@ -228,13 +289,13 @@ struct CullRayEu {
     }
     // Branch-free: every term is evaluated, so several elements' tests can
     // be interleaved; the decision is the same boolean function.
-    HD bool candidate(const Bound &b) const {
-        double sx = (double)b.cx - px, sy = (double)b.cy - py, sz = (double)b.cz - pz;
-        double r = b.r;
+    template <class B> HD bool candidate(const B &b) const {
+        double sx = b_cx(b) - px, sy = b_cy(b) - py, sz = b_cz(b) - pz;
+        double r = b_r(b);
         double t = sx * dx + sy * dy + sz * dz;
         double qx = sx - t * dx, qy = sy - t * dy, qz = sz - t * dz;
         bool hit = !(t < -r) && qx * qx + qy * qy + qz * qz <= b_r2(b);  // not wholly behind, and within r
-        return b.r < 0 || !valid || hit;
+        return r < 0 || !valid || hit;
     }
 };
 
@@ -270,8 +331,8 @@ struct CullRayHy {
     HD bool toward_tangent_point(double tx, double ty) const;
     // Branch-free (the same boolean function as the sequence of tests it
     // describes), so several elements' tests can be interleaved.
-    HD bool candidate(const Bound &b) const {
-        double rx = (double)b.cx - px, ry = (double)b.cy - py;
+    template <class B> HD bool candidate(const B &b) const {
+        double rx = b_cx(b) - px, ry = b_cy(b) - py;
         double delta = -rx * uy + ry * ux;  // distance from the vertical plane
         double d2p = delta * delta;
         double rp2 = b_r2(b) - d2p;  // squared radius of the plane's cut of the outer sphere
@@ -288,7 +349,8 @@ struct CullRayHy {
         //    crossings behind the start.
         bool behind = sig < 0.0 && sig * sig > rp2;
         // horo = 2 marks a tree node whose subtree holds a horoball: no "behind" cull
-        if (behind && b.horo > 0) behind = b.horo < 1.5 && !toward_tangent_point(b.cx, b.cy);
+        float horo = b_horo(b);
+        if (behind && horo > 0) behind = horo < 1.5f && !toward_tangent_point(b_cx(b), b_cy(b));
         cull = cull || behind;
         // 3. disjoint circles: d2 > (rg + rp)^2  <=>  d2 - rg^2 - rp^2 > 2 rg rp
         double o = d2 - rg2 - rp2;
@@ -301,7 +363,7 @@ struct CullRayHy {
         double rpi2 = b_ri2(b) - d2p;
         double in2 = rg2 + rpi2 - d2;
         cull = cull || (rpi2 > rg2 && in2 > 0.0 && in2 * in2 > four_rg2 * rpi2);
-        return b.r < 0 || !valid || !cull;
+        return b_r(b) < 0 || !valid || !cull;
     }
 };
 
@@ -356,6 +418,9 @@ template <class G> HD WaveScratch<G> *wave_scratch_lds() {
 // is a dependent chain (node load -> FP64 test -> next node) at low
 // occupancy, and nodes holding a horoball cannot use the "behind" cull. The
 // flat scan is the default.
+#if defined(HT_CULL_TREE) && defined(HT_SCALAR_BOUNDS)
+#error "HT_CULL_TREE walks the LDS bounds; HT_SCALAR_BOUNDS has none"
+#endif
 #ifdef HT_CULL_TREE
 static constexpr bool TREE_OK = BOUNDS_CAP >= 64 && BOUNDS_CAP <= 1024;
 #else
@@ -456,6 +521,7 @@ HD unsigned long long hit_key(float d, usize i) {
 // elements in FP32 programs, the element-by-element scan otherwise.
 template <class T> struct TopScan {
     static HD void prepare(const u8 *) {}
+    static HD void fill_scalar(const u8 *) {}
     template <class G>
     static HD real detect(const u8 *self, Context *ctx, typename T::Cache *cache, Light<G> *light) {
         return T::detect(self, ctx, cache, light);
@@ -476,6 +542,9 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     // All threads of the block, before any of them diverges; the caller
     // synchronises the block afterwards.
     static HD void prepare(const u8 *self) {
+#ifdef HT_SCALAR_BOUNDS
+        (void)self;  // the bounds are in ht_scalar_bounds (fill_scalar)
+#else
         if constexpr (CULL) {
             usize n = V::len(self);
             if (n > (usize)BOUNDS_CAP) return;
@@ -490,6 +559,22 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 }
             }
         }
+#endif
+    }
+
+    // The bounds of the scalar-load scan, from any grid (grid-stride loop);
+    // a scene with more elements than the array holds is scanned unculled.
+    static HD void fill_scalar(const u8 *self) {
+#ifdef HT_SCALAR_BOUNDS
+        if constexpr (CULL) {
+            usize n = V::len(self);
+            if (n > (usize)BOUNDS_CAP) return;
+            for (uint i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint)n; i += gridDim.x * blockDim.x)
+                ht_scalar_bounds[i] = widen(element_bound<E, typename E::Map>(V::elem(self, i)));
+        }
+#else
+        (void)self;
+#endif
     }
 
     // All threads of the block (see TreeLds).
@@ -622,13 +707,63 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         }
     }
 
+    // The sphere tree's leaves in walk order, each with its cull decision,
+    // into `append` (the flat scan's pair queue). HT_CULL_TREE only.
+    template <class CR, class Append> static HD void walk_tree(const CR &cr, Append &append) {
+        const Bound *bounds = bounds_lds<OV>();
+        // Stackless depth-first walk of the sphere tree, one node per
+        // lane per step: descend to 2k when node k passes, else move
+        // to the next subtree (drop the trailing 1 bits, then +1);
+        // node 0 = done. Lanes walk different nodes on the same code.
+        constexpr uint P = BOUNDS_CAP;
+        const TreeLds *t = tree_lds<OV>();
+        uint node = 1;
+        for (;;) {
+            bool live = node != 0;
+            if (__ballot(live) == 0) break;
+            uint elem = 0;
+            bool cand = false;
+            if (live) {
+                Bound bb;
+                if (node >= P) {
+                    unsigned short e = t->order[node - P];
+                    elem = e;
+                    if (e != NO_ELEMENT) {
+                        bb = bounds[e];
+                    } else {
+                        bb.r = -2.f;
+                    }
+                } else {
+                    bb = MakeBound<Bound>::node(t->node[node], t->horo[node] != 0);
+                }
+                bool pass = bb.r != -2 && cr.candidate(bb);
+                if (node >= P) {
+                    cand = pass;
+                }
+                if (pass && node < P) {
+                    node = 2 * node;
+                } else {
+                    node >>= __builtin_ctz(~node);
+                    if (node) node += 1;
+                }
+            }
+            append(elem, cand);
+        }
+    }
+
     static HD real detect(const u8 *self, Context *ctx, typename OV::Cache *cache, Light<G> *light) {
         if constexpr (!CULL) {
             return OV::detect(self, ctx, cache, light);
         } else {
             usize n = V::len(self);
             if (n > (usize)BOUNDS_CAP) return OV::detect(self, ctx, cache, light);
-            const Bound *bounds = bounds_lds<OV>();
+#ifdef HT_SCALAR_BOUNDS
+            const ConstBoundS *sb = (const ConstBoundS *)ht_scalar_bounds;
+            auto bound = [&](usize e) { return load_bound(sb + e); };
+#else
+            const Bound *lb = bounds_lds<OV>();
+            auto bound = [&](usize e) -> const Bound & { return lb[e]; };
+#endif
             WaveScratch<G> *ws = wave_scratch_lds<G>() + (threadIdx.x / WAVE);
             uint lane = __lane_id();
             uint orig = ctx->hasher.state;
@@ -665,44 +800,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             bool tree = false;
             if constexpr (TREE_OK) tree = n >= TREE_MIN_ELEMENTS;
             if (tree) {
-                // Stackless depth-first walk of the sphere tree, one node per
-                // lane per step: descend to 2k when node k passes, else move
-                // to the next subtree (drop the trailing 1 bits, then +1);
-                // node 0 = done. Lanes walk different nodes on the same code.
-                constexpr uint P = BOUNDS_CAP;
-                const TreeLds *t = tree_lds<OV>();
-                uint node = 1;
-                for (;;) {
-                    bool live = node != 0;
-                    if (__ballot(live) == 0) break;
-                    uint elem = 0;
-                    bool cand = false;
-                    if (live) {
-                        Bound bb;
-                        if (node >= P) {
-                            unsigned short e = t->order[node - P];
-                            elem = e;
-                            if (e != NO_ELEMENT) {
-                                bb = bounds[e];
-                            } else {
-                                bb.r = -2.f;
-                            }
-                        } else {
-                            bb = MakeBound<Bound>::node(t->node[node], t->horo[node] != 0);
-                        }
-                        bool pass = bb.r != -2 && cr.candidate(bb);
-                        if (node >= P) {
-                            cand = pass;
-                        }
-                        if (pass && node < P) {
-                            node = 2 * node;
-                        } else {
-                            node >>= __builtin_ctz(~node);
-                            if (node) node += 1;
-                        }
-                    }
-                    append(elem, cand);
-                }
+                if constexpr (TREE_OK) walk_tree(cr, append);
             } else {
                 // CULL_UNROLL independent cull tests per step (instruction-level
                 // parallelism, traded against registers and so occupancy).
@@ -710,11 +808,11 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 for (; e + CULL_UNROLL <= n; e += CULL_UNROLL) {
                     bool c[CULL_UNROLL];
 #pragma unroll
-                    for (int u = 0; u < CULL_UNROLL; ++u) c[u] = cr.candidate(bounds[e + u]);
+                    for (int u = 0; u < CULL_UNROLL; ++u) c[u] = cr.candidate(bound(e + u));
 #pragma unroll
                     for (int u = 0; u < CULL_UNROLL; ++u) append(e + u, c[u]);
                 }
-                for (; e < n; ++e) append(e, cr.candidate(bounds[e]));
+                for (; e < n; ++e) append(e, cr.candidate(bound(e)));
             }
             if (qcount > 0) {
                 wave_sync();

@@ -42,6 +42,7 @@ struct ht_prog {
     hipFunction_t render_passes = nullptr;
     hipFunction_t render_cached = nullptr;  // null unless the scene type supports the radiance cache
     hipFunction_t render_batch = nullptr;   // batch programs (several scene types, one persistent kernel)
+    hipFunction_t bounds_fill = nullptr;    // scalar-bounds programs: fills the bounds before each render launch
 };
 
 struct ht_buf {
@@ -53,6 +54,16 @@ struct ht_buf {
 namespace ht_internal {
 
 inline void ctx_retain(ht_ctx *ctx) { ctx->refs.fetch_add(1); }
+
+// Scalar-bounds programs: writes the scene's element bounds into the
+// program's bounds array on the context's stream, ahead of a render launch
+// (4 blocks; the kernel strides over any element count). No-op otherwise.
+inline int fill_bounds(ht_prog *prog, void *scene_ptr) {
+    if (!prog->bounds_fill) return HT_OK;
+    void *args[] = {&scene_ptr};
+    HIP_TRY(hipModuleLaunchKernel(prog->bounds_fill, 4, 1, 1, 256, 1, 1, 0, prog->ctx->stream, args, nullptr));
+    return HT_OK;
+}
 inline void ctx_release(ht_ctx *ctx) {
     if (ctx->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(ctx->device);

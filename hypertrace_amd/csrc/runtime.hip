@@ -450,6 +450,7 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
     p->render_passes = find("render_passes");
     p->render_cached = find("render_cached");
     p->render_batch = find("render_batch");
+    p->bounds_fill = find("bounds_fill");
     if (!(p->render && p->render_passes) && !p->render_batch) {
         (void)hipModuleUnload(p->module);
         delete p;
@@ -571,6 +572,7 @@ int launch_render(ht_prog *prog, hipFunction_t fn, uint32_t w, uint32_t h, ht_bu
     uint32_t block = 256;
     uint32_t grid = (n + block - 1) / block;
     void *sp = scene->ptr, *cp = canvas->ptr, *pp = seeds->ptr;
+    if ((rc = ht_internal::fill_bounds(prog, sp))) return rc;
     void *args_pass[] = {&w, &h, &sp, &cp, &pp};
     void *args_passes[] = {&w, &h, &sp, &cp, &pp, &passes};
     HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, ctx->stream,

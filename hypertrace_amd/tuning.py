@@ -1,13 +1,16 @@
 """Autotuner of the render program's code-generation knobs for a workload.
 
-Two compile-time knobs of the scene program trade instruction-level
-parallelism against registers and so against resident waves:
-HT_CULL_UNROLL (cull tests per step of the nearest-hit scan) and
-HT_WAVES_PER_EU (minimum waves per SIMD the register allocation must allow;
-above 4 the compiler spills a little). Every setting renders bit-identical
-images (only the schedule differs), so the tuner simply times one frame of
-the caller's workload per candidate on the GPU and keeps the fastest. Small
-frames (few waves per SIMD) prefer fewer registers less than large ones.
+Three compile-time knobs of the scene program: HT_CULL_UNROLL (cull tests
+per step of the nearest-hit scan) and HT_WAVES_PER_EU (minimum waves per
+SIMD the register allocation must allow; above 4 the compiler spills a
+little) trade instruction-level parallelism against registers and so
+against resident waves; HT_SCALAR_BOUNDS moves the element bounds from LDS
+(FP32, widened per lane) to a global FP64 array read through the scalar
+cache, which frees the LDS that limits occupancy on scenes with many
+elements. Every setting renders bit-identical images (only the schedule
+differs), so the tuner simply times one frame of the caller's workload per
+candidate on the GPU and keeps the fastest. Small frames (few waves per
+SIMD) prefer fewer registers less than large ones.
 """
 from __future__ import annotations
 
@@ -17,19 +20,27 @@ from typing import Dict, Optional, Sequence, Tuple
 
 from .processing import STRICT_FP, Canvas, Context, EntityBuffer, Render
 
-CANDIDATES: Tuple[Tuple[int, int], ...] = ((4, 4), (4, 5), (2, 5), (4, 6))
+# (cull unroll, waves per SIMD, scalar bounds)
+CANDIDATES: Tuple[Tuple[int, int, bool], ...] = ((4, 4, False), (4, 5, False), (2, 5, False), (4, 6, False),
+                                                 (4, 4, True), (4, 5, True))
 
 
-def options(unroll: int, waves: int, base: str = STRICT_FP) -> str:
-    return f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
+def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False) -> str:
+    opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
+    return opts + " -DHT_SCALAR_BOUNDS=1" if scalar_bounds else opts
+
+
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool]:
+    unroll, waves = int(candidate[0]), int(candidate[1])
+    return unroll, waves, bool(candidate[2]) if len(candidate) > 2 else False
 
 
 def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mode: int = 1, seed: int = 0,
-                candidates: Sequence[Tuple[int, int]] = CANDIDATES, reps: int = 3, base: str = STRICT_FP,
+                candidates: Sequence[Sequence] = CANDIDATES, reps: int = 3, base: str = STRICT_FP,
                 cache_path: Optional[str] = None) -> Dict:
     """Times `spp` passes of render_passes (HIP events, median of `reps`) for
-    each (unroll, waves) candidate; returns {"best": {...}, "table": [...]}
-    with the program options of the best."""
+    each (unroll, waves[, scalar bounds]) candidate; returns {"best": {...},
+    "table": [...]} with the program options of the best."""
     key = f"gfx950:{scene.cpp()}:{width}x{height}:{spp}:{base}"
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as f:
@@ -38,8 +49,9 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             return known[key]
     buf = EntityBuffer(ctx, scene)
     table = []
-    for unroll, waves in candidates:
-        opts = options(unroll, waves, base)
+    for cand in candidates:
+        unroll, waves, sb = _knobs(cand)
+        opts = options(unroll, waves, base, sb)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -49,8 +61,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             ctx.timer_begin()
             r.render_passes(buf, c, spp)
             times.append(ctx.timer_end())
-        table.append({"unroll": unroll, "waves_per_eu": waves, "ms": sorted(times)[len(times) // 2],
-                      "options": opts})
+        table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb,
+                      "ms": sorted(times)[len(times) // 2], "options": opts})
     best = min(table, key=lambda t: t["ms"])
     result = {"best": best, "table": table}
     if cache_path:

@@ -9,7 +9,7 @@ import pytest
 from oracle import mlp as om
 from hypertrace_amd import scenes
 from hypertrace_amd.native import HtError
-from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
+from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render, program_options
 from hypertrace_amd.radiance_cache import CachedRender, RadianceCache, init_params, pack_params, unpack_params
 from hypertrace_amd.types import X32_F32, X32_F64
 
@@ -125,6 +125,27 @@ def test_cached_render_queues(ctx):
     assert 0 < q < 96 * 72 and 0 < r < 96 * 72 // 4 and q + r <= 96 * 72
     img = canvas.load_image()
     assert np.all(np.isfinite(img)) and np.all(img[..., 3] == 1.0)
+
+
+def test_cached_render_scalar_bounds_identical(ctx):
+    """The radiance-cache kernel of a scalar-bounds program (bounds filled by
+    the program's bounds_fill kernel before the launch) renders the same
+    pass as the default program: canvas after inference, seeds, queue
+    counts."""
+    wl = scenes.hyperbolic_cells(96, 72, 1)
+    buf = EntityBuffer(ctx, wl.scene)
+    out = []
+    for fp in (None, program_options(scalar_bounds=True)):
+        cache = RadianceCache(ctx, 96 * 72, 65536, 1024)
+        cr = CachedRender(ctx, wl.scene, cache, steps_per_pass=0, batch=1024, fp_options=fp)
+        canvas = Canvas(ctx, (96, 72), wl.seed_mode, wl.seed)
+        cr.render_pass(buf, canvas, train=False)
+        cr.render_pass(buf, canvas, train=False)
+        ctx.sync()
+        out.append((canvas.load_image().copy(), canvas.load_seeds().copy(), cache.counts()))
+    np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
 
 
 def test_cached_render_needs_a_supported_scene(ctx):

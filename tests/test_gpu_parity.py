@@ -181,24 +181,34 @@ def test_pipeline_fused_post_equals_separate(ctx):
 NO_CULL = NO_CULL_FP
 
 
-@pytest.mark.parametrize("name,full,tree", [("eu_ball", False, False), ("eu_unit", False, False),
-                                            ("hy_cells", False, False), ("hy_cells", True, False),
-                                            ("eu_example", False, False), ("hy_example", False, False),
-                                            ("sdf", False, False), ("sdf_1024", False, False),
-                                            ("hy_cells", False, True), ("sdf_1024", False, True),
-                                            ("hy_cells", True, "unroll2_waves5")])
-def test_culled_scan_identical_to_plain_scan(ctx, name, full, tree):
-    """The culled, compacted nearest-hit scan (flat, or walking the sphere
-    tree with -DHT_CULL_TREE) must give bit-identical canvas and seeds to
-    the element-by-element scan."""
+# Program options of the culled scan's variants (all must render the same
+# images): the default flat scan over LDS bounds, the sphere tree, a render
+# tuner setting that spills, and the scalar-cache bounds.
+CULLED_VARIANTS = {
+    "flat": None,
+    "tree": STRICT_FP + " -DHT_CULL_TREE=1",
+    "unroll2_waves5": STRICT_FP + " -DHT_CULL_UNROLL=2 -DHT_WAVES_PER_EU=5",
+    "scalar_bounds": STRICT_FP + " -DHT_SCALAR_BOUNDS=1",
+    "scalar_bounds_u2_w5": STRICT_FP + " -DHT_CULL_UNROLL=2 -DHT_WAVES_PER_EU=5 -DHT_SCALAR_BOUNDS=1",
+}
+
+
+@pytest.mark.parametrize("name,full,variant", [
+    ("eu_ball", False, "flat"), ("eu_unit", False, "flat"), ("hy_cells", False, "flat"), ("hy_cells", True, "flat"),
+    ("eu_example", False, "flat"), ("hy_example", False, "flat"), ("sdf", False, "flat"),
+    ("sdf_1024", False, "flat"), ("hy_cells", False, "tree"), ("sdf_1024", False, "tree"),
+    ("hy_cells", True, "unroll2_waves5"), ("eu_ball", False, "scalar_bounds"), ("hy_cells", True, "scalar_bounds"),
+    ("hy_example", False, "scalar_bounds"), ("sdf_1024", False, "scalar_bounds"),
+    ("hy_cells", False, "scalar_bounds_u2_w5")])
+def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
+    """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
+    bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
+    bit-identical canvas and seeds to the element-by-element scan."""
     wl = scenes.hyperbolic_cells() if full else WORKLOADS[name]()
-    print(f"case {name} full={full} tree={tree}", flush=True)
+    print(f"case {name} full={full} variant={variant}", flush=True)
     buf = EntityBuffer(ctx, wl.scene)
     out = []
-    if tree == "unroll2_waves5":  # a tuner setting that spills: same images
-        culled = STRICT_FP + " -DHT_CULL_UNROLL=2 -DHT_WAVES_PER_EU=5"
-    else:
-        culled = (STRICT_FP + " -DHT_CULL_TREE=1") if tree else None
+    culled = CULLED_VARIANTS[variant]
     for fp in (culled, NO_CULL):
         r = Render(ctx, wl.scene, fp_options=fp)
         c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
