@@ -246,6 +246,17 @@ def cache_leg(args):
     return out
 
 
+def fixed_knobs(text, base=None):
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb]" (no timing)."""
+    from hypertrace_amd.processing import STRICT_FP
+    from hypertrace_amd.tuning import options
+    knobs = text.split(",")
+    unroll, waves, sb = int(knobs[0]), int(knobs[1]), knobs[2:] == ["sb"]
+    return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb,
+                     "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb)},
+            "table": [], "fixed": True}
+
+
 def sdf_leg(args):
     """The sphere-traced scene at 1920x1080: 1024 SDF shapes, 256 spp, depth
     16, Russian roulette from the 3rd hit, strict FP32: Mpaths/s of one frame
@@ -262,7 +273,10 @@ def sdf_leg(args):
     with Context(0, X32_F32) as ctx:
         # the render tuner over 4-spp frames (all settings render the same images)
         from hypertrace_amd.tuning import tune_render
-        tuning = tune_render(ctx, wl.scene, W, H, 4, wl.seed_mode, wl.seed, base=opts)
+        if args.sdf_knobs:
+            tuning = fixed_knobs(args.sdf_knobs, opts)
+        else:
+            tuning = tune_render(ctx, wl.scene, W, H, 4, wl.seed_mode, wl.seed, base=opts)
         r = Render(ctx, wl.scene, fp_options=tuning["best"]["options"])
         b = EntityBuffer(ctx, wl.scene)
         c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
@@ -365,6 +379,8 @@ def main():
     ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb]",
                     help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
                          "instead of tuning (keeps a profiled trace to one program)")
+    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb]",
+                    help="the same for the SDF leg's program")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "
                          "the known-size copy that scales FETCH_SIZE / WRITE_SIZE counter readings")
@@ -413,13 +429,9 @@ def main():
         # chosen program: strict IEEE FP32, culled scan, with the unroll /
         # waves-per-SIMD setting the tuner picks for this frame (all settings
         # render bit-identical images)
-        from hypertrace_amd.tuning import options as knob_options, tune_render
+        from hypertrace_amd.tuning import tune_render
         if args.render_knobs:
-            knobs = args.render_knobs.split(",")
-            unroll, waves, sb = int(knobs[0]), int(knobs[1]), knobs[2:] == ["sb"]
-            tuning = {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb,
-                               "options": knob_options(unroll, waves, scalar_bounds=sb)},
-                      "table": [], "fixed": True}
+            tuning = fixed_knobs(args.render_knobs)
         else:
             tuning = tune_render(ctx, wl.scene, W, H, SPP, wl.seed_mode, wl.seed)
         extra["render_tuning"] = tuning

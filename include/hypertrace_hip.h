@@ -55,8 +55,16 @@ int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int 
  * "-ffp-contract=off -DHT_STRICT_FP=1": every operation rounded on its own,
  * IEEE division and square root, FP32 transcendentals correctly rounded.
  * "-fno-hip-fp32-correctly-rounded-divide-sqrt" is the fast mode: FMA
- * contraction, reciprocal division, the FP32 device libm. No reference
- * counterpart beyond ocl::Program build options. */
+ * contraction, reciprocal division, the FP32 device libm. Further macros
+ * select schedules that render the same images: -DHT_CULL_UNROLL=n (cull
+ * tests per scan step, default 4), -DHT_WAVES_PER_EU=w (register cap,
+ * default 4), -DHT_SCALAR_BOUNDS=1 (the scan's bounds in a global FP64
+ * array read through the scalar cache, filled by the program's bounds_fill
+ * kernel, which ht_render / ht_render_passes / ht_render_cached launch
+ * first on the context's stream), -DHT_CULL_TREE=1 (sphere-tree walk); and
+ * methods that do not: -DHT_RR_DEPTH=d (Russian roulette), -DHT_NO_CULL=1
+ * (the element-by-element scan, same images, for comparisons). No
+ * reference counterpart beyond ocl::Program build options. */
 int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
                      const char *const *names, const char *const *contents, int double_precision,
                      int address_width, const char *fp_options, ht_prog **out);
