@@ -35,7 +35,7 @@ def build_library() -> str:
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     tmp = LIB + ".tmp"
     run([HIPCC, *HIP_FLAGS, "-shared", os.path.join(PKG, "csrc", "runtime.hip"), os.path.join(PKG, "csrc", "cache.hip"),
-         os.path.join(PKG, "csrc", "batch.hip"),
+         os.path.join(PKG, "csrc", "batch.hip"), os.path.join(PKG, "csrc", "wavefront.hip"),
          "-L/opt/rocm/lib", "-lhiprtc",
          "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
     os.replace(tmp, LIB)

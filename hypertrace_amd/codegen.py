@@ -86,18 +86,25 @@ def cache_supported(scene: SceneImpl) -> bool:
         isinstance(e, Mapped) and isinstance(e.inner, Covered) for e in obj.items)
 
 
-def root_source(scene: SceneImpl, cfg: Config) -> str:
+def root_source(scene: SceneImpl, cfg: Config, wavefront: bool = False) -> str:
+    """Root source of a scene's render program: the render kernels, the
+    radiance-cache kernel where the scene supports it, and with `wavefront`
+    the kernels of the wavefront form (ht_render_wavefront)."""
     cached = cache_supported(scene)
     lines = [
         "// Generated render program for one scene type.",
         f"#define HT_BOUNDS_CAP {bounds_cap(scene)}",
         "#include <ht/cache.hh>" if cached else "#include <ht/render.hh>",
-        f"typedef {scene.cpp()} Scene;",
     ]
+    if wavefront:
+        lines.append("#include <ht/wavefront.hh>")
+    lines.append(f"typedef {scene.cpp()} Scene;")
     lines += layout_asserts(scene, cfg)
     lines.append("HT_DEFINE_RENDER_KERNELS(Scene)")
     if cached:
         lines.append("HT_DEFINE_CACHED_KERNEL(Scene)")
+    if wavefront:
+        lines.append("HT_DEFINE_WAVEFRONT_KERNELS(Scene)")
     return "\n".join(lines) + "\n"
 
 

@@ -697,8 +697,11 @@ extern "C" int ht_render_cached(ht_prog *prog, ht_cache *c, uint32_t w, uint32_t
     void *args[] = {&w, &h, &sp, &cp, &dp, &pass_index, &c->counters, &c->q_pixel, &c->q_feat, &c->q_thr, &qcap,
                     &c->t_feat, &c->t_target, &tcap, &train_every, &query_hop};
     uint32_t grid = (uint32_t)((npx + 255) / 256);
-    if ((rc = ht_internal::fill_bounds(prog, sp))) return rc;
-    HIP_TRY(hipModuleLaunchKernel(prog->render_cached, grid, 1, 1, 256, 1, 1, 0, ctx->stream, args, nullptr));
+    {
+        std::lock_guard<std::mutex> lock(prog->launch_mu);
+        if ((rc = ht_internal::fill_bounds(prog, sp))) return rc;
+        HIP_TRY(hipModuleLaunchKernel(prog->render_cached, grid, 1, 1, 256, 1, 1, 0, ctx->stream, args, nullptr));
+    }
     uint32_t blocks = min((uint32_t)((npx + 63) / 64), 2048u);
     hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, ctx->stream, c->image, c->params, c->counters, 0u,
                        c->qcap, c->q_feat, c->q_pixel, c->q_thr, (float4 *)canvas->ptr, nullptr);

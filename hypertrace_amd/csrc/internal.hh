@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <mutex>
 #include <string>
 
 #include "../../include/hypertrace_hip.h"
@@ -43,6 +44,18 @@ struct ht_prog {
     hipFunction_t render_cached = nullptr;  // null unless the scene type supports the radiance cache
     hipFunction_t render_batch = nullptr;   // batch programs (several scene types, one persistent kernel)
     hipFunction_t bounds_fill = nullptr;    // scalar-bounds programs: fills the bounds before each render launch
+    // Wavefront form (programs whose root source adds the wavefront kernels):
+    // kernels, record sizes {ray words, accumulator words, hit words, keys,
+    // counter words, HOPS} and the SoA / queue workspace, grown on demand.
+    hipFunction_t wf_generate = nullptr, wf_extend = nullptr, wf_shade = nullptr;
+    unsigned wf_layout[6] = {0, 0, 0, 0, 0, 0};
+    void *wf_mem = nullptr;
+    size_t wf_bytes = 0;
+    // Held from a scalar-bounds program's bounds_fill enqueue to its render
+    // enqueue, so callers sharing the program from several threads cannot
+    // interleave another scene's fill between the two (stream order does
+    // the rest).
+    std::mutex launch_mu;
 };
 
 struct ht_buf {

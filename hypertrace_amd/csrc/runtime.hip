@@ -451,6 +451,22 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
     p->render_cached = find("render_cached");
     p->render_batch = find("render_batch");
     p->bounds_fill = find("bounds_fill");
+    p->wf_generate = find("wf_generate");
+    p->wf_extend = find("wf_extend");
+    p->wf_shade = find("wf_shade");
+    if (p->wf_generate && p->wf_extend && p->wf_shade) {
+        hipDeviceptr_t d = nullptr;
+        size_t bytes = 0;
+        e = hipModuleGetGlobal(&d, &bytes, p->module, "ht_wf_layout");
+        if (e == hipSuccess && bytes == sizeof(p->wf_layout)) e = hipMemcpyDtoH(p->wf_layout, d, bytes);
+        if (e != hipSuccess || bytes != sizeof(p->wf_layout)) {
+            (void)hipModuleUnload(p->module);
+            delete p;
+            return fail(HT_ERR_BACKEND, "program has wavefront kernels but no readable ht_wf_layout");
+        }
+    } else {
+        p->wf_generate = p->wf_extend = p->wf_shade = nullptr;
+    }
     if (!(p->render && p->render_passes) && !p->render_batch) {
         (void)hipModuleUnload(p->module);
         delete p;
@@ -479,6 +495,10 @@ extern "C" int ht_prog_precompile(const char *root_name, const char *root_src, i
 extern "C" void ht_prog_destroy(ht_prog *prog) {
     if (!prog) return;
     (void)hipSetDevice(prog->ctx->device);
+    if (prog->wf_mem) {
+        (void)hipStreamSynchronize(prog->ctx->stream);
+        (void)hipFree(prog->wf_mem);
+    }
     (void)hipModuleUnload(prog->module);
     (void)hipGetLastError();
     ht_ctx *ctx = prog->ctx;
@@ -572,6 +592,7 @@ int launch_render(ht_prog *prog, hipFunction_t fn, uint32_t w, uint32_t h, ht_bu
     uint32_t block = 256;
     uint32_t grid = (n + block - 1) / block;
     void *sp = scene->ptr, *cp = canvas->ptr, *pp = seeds->ptr;
+    std::lock_guard<std::mutex> lock(prog->launch_mu);
     if ((rc = ht_internal::fill_bounds(prog, sp))) return rc;
     void *args_pass[] = {&w, &h, &sp, &cp, &pp};
     void *args_passes[] = {&w, &h, &sp, &cp, &pp, &passes};

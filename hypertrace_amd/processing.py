@@ -252,11 +252,14 @@ def precompile(scene: Optional[SceneImpl], config: Config, name: str = "render.h
 
 class Render:
     """Render program of a scene type (render.rs:8-119). `render` runs one
-    synchronous pass; `render_passes` queues several passes in one launch."""
+    synchronous pass; `render_passes` queues several passes in one launch;
+    with `wavefront`, `render_wavefront` computes the same passes as queued
+    extend / shade kernels (ht_render_wavefront)."""
 
-    def __init__(self, ctx: Context, scene: SceneImpl, fp_options: Optional[str] = None):
+    def __init__(self, ctx: Context, scene: SceneImpl, fp_options: Optional[str] = None, wavefront: bool = False):
         self.ctx = ctx
-        self.program = Program(ctx, scene, fp_options=fp_options)
+        source = codegen.root_source(scene, ctx.config, wavefront=True) if wavefront else None
+        self.program = Program(ctx, scene, fp_options=fp_options, source=source)
 
     def render(self, scene_buffer: EntityBuffer, canvas: Canvas) -> None:
         native.check(native.lib().ht_render(self.program.handle, canvas.width, canvas.height, scene_buffer.handle,
@@ -267,6 +270,12 @@ class Render:
         native.check(native.lib().ht_render_passes(self.program.handle, canvas.width, canvas.height,
                                                    scene_buffer.handle, canvas.image.handle, canvas.seeds.handle,
                                                    passes))
+        canvas.add_pass(passes)
+
+    def render_wavefront(self, scene_buffer: EntityBuffer, canvas: Canvas, passes: int) -> None:
+        native.check(native.lib().ht_render_wavefront(self.program.handle, canvas.width, canvas.height,
+                                                      scene_buffer.handle, canvas.image.handle,
+                                                      canvas.seeds.handle, passes))
         canvas.add_pass(passes)
 
 

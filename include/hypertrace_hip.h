@@ -100,6 +100,19 @@ int ht_render(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canv
 int ht_render_passes(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas_f32x4,
                      ht_buf *seeds_u32, uint32_t passes);
 
+/* The same result as ht_render_passes (canvas and seeds bit for bit), as a
+ * wavefront: paths live in SoA state in device memory and advance one hit
+ * per iteration through an extend kernel (nearest hit; hits sorted into
+ * queues by the material component they take next) and a shade kernel
+ * (material step in queue order, regeneration of the pixel's next sample).
+ * The program's root source must add HT_DEFINE_WAVEFRONT_KERNELS(Scene)
+ * (codegen.root_source(..., wavefront=True)); its workspace (4 bytes x
+ * (path-state + hit-record words + 6) per pixel, ~170 B for the cell scene)
+ * is kept by the program and grown on demand. Synchronous. No reference counterpart: the form the
+ * megakernel is measured against. */
+int ht_render_wavefront(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas_f32x4,
+                        ht_buf *seeds_u32, uint32_t passes);
+
 /* Post passes (asynchronous until a read). image, out: float4[w*h];
  * rgba8: uchar4[w*h].
  * ht_extract: image = canvas / (float)passes   (processing/src/canvas.rs:126-135)
@@ -187,9 +200,10 @@ int ht_cache_train(ht_cache *cache, uint32_t steps, uint32_t batch, float lr);
  * A batch program is built with ht_prog_build_ex from a root source that
  * instantiates HT_DEFINE_BATCH_KERNEL(Scene0, Scene1) (codegen.batch_source);
  * its block size is the kernel's launch bound; `tile` (pixels per work item,
- * a multiple of the block size) is chosen at ht_batch_create. Per pixel the work equals ht_render_passes of the
- * job's scene with at most `depth` hits per path, so canvases and seeds are
- * bit-identical to rendering each job alone. */
+ * a multiple of the block size) is chosen at ht_batch_create. Per pixel the
+ * work equals ht_render_passes of the job's scene with at most `depth` hits
+ * per path, so canvases and seeds are bit-identical to rendering each job
+ * alone. */
 typedef struct ht_batch ht_batch;
 typedef struct ht_batch_job {
     uint32_t type;             /* 0 or 1: Scene0 or Scene1 of the batch program */
