@@ -310,6 +310,53 @@ def sdf_leg(args):
                                "note": "independent seeds: the RMSE is MC noise of both images plus any FP32 bias"}}
 
 
+def wavefront_leg(args):
+    """The cell frame (640x480, 64 spp, 192 primitives) through the wavefront
+    form (ht_render_wavefront: SoA path state in HBM, extend / material-sorted
+    shade kernels) next to the megakernel's render_passes on the same
+    program options: time per frame, and whether canvas and seeds agree bit
+    for bit. The wavefront call is synchronous (it reads the live-path count
+    back every few iterations), so its time is wall-clock."""
+    import numpy as np
+
+    from hypertrace_amd import scenes
+    from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
+    from hypertrace_amd.types import X32_F32
+
+    wl = scenes.hyperbolic_cells()
+    W, H, SPP = wl.width, wl.height, wl.spp
+    with Context(0, X32_F32) as ctx:
+        buf = EntityBuffer(ctx, wl.scene)
+        mega = Render(ctx, wl.scene)
+        wave = Render(ctx, wl.scene, wavefront=True)
+        results = {}
+        for name, run in (("megakernel", lambda c: mega.render_passes(buf, c, SPP)),
+                          ("wavefront", lambda c: wave.render_wavefront(buf, c, SPP))):
+            c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+            run(c)  # warm-up, and the frame compared below
+            ctx.sync()
+            img, seeds = c.load_image().copy(), c.load_seeds().copy()
+            times = []
+            for _ in range(args.wavefront_steps):
+                c2 = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+                ctx.sync()
+                t0 = time.perf_counter()
+                run(c2)
+                ctx.sync()
+                times.append(time.perf_counter() - t0)
+            results[name] = {"ms_per_frame": min(times) * 1e3, "value": W * H * SPP / min(times) / 1e6,
+                             "unit": "Mpaths/s", "img": img, "seeds": seeds}
+        same = bool(np.array_equal(results["megakernel"]["img"].view(np.uint32),
+                                   results["wavefront"]["img"].view(np.uint32)) and
+                    np.array_equal(results["megakernel"]["seeds"], results["wavefront"]["seeds"]))
+    for r in results.values():
+        del r["img"], r["seeds"]
+    return {"workload": f"hyperbolic_cells {W}x{H}, {SPP} spp, 192 primitives (strict FP32, culled scan)",
+            "megakernel": results["megakernel"], "wavefront": results["wavefront"],
+            "bit_identical": same,
+            "wavefront_over_megakernel": results["wavefront"]["ms_per_frame"] / results["megakernel"]["ms_per_frame"]}
+
+
 def batch_leg(args):
     """32 uneven jobs (half Euclidean, half hyperbolic; 128x128..1024x768
     log-uniform, spp U{1..64}, depth U{4..12}, 8..256 primitives of the
@@ -372,6 +419,8 @@ def main():
     ap.add_argument("--cache-max-spp", type=int, default=1024)
     ap.add_argument("--no-sdf", action="store_true", help="skip the 1080p SDF leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batched leg")
+    ap.add_argument("--no-wavefront", action="store_true", help="skip the wavefront-form leg")
+    ap.add_argument("--wavefront-steps", type=int, default=2)
     ap.add_argument("--sdf-spp", type=int, default=256)
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
@@ -519,6 +568,11 @@ def main():
             out["batch_32_jobs"] = batch_leg(args)
         except Exception as e:
             out["batch_32_jobs"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_wavefront:
+        try:
+            out["wavefront_cells"] = wavefront_leg(args)
+        except Exception as e:
+            out["wavefront_cells"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_sdf:
         try:
             out["sdf_1080p"] = sdf_leg(args)

@@ -3,27 +3,28 @@
 // split into kernels over queues of live paths instead of one thread
 // looping over a pixel's samples.
 //
-// wf_generate  every pixel's first path (camera ray) into the SoA path
-//              state; the extend queue of iteration 0 = all pixels.
+// wf_generate  every pixel's first path (camera ray): extend queue 0 holds
+//              all pixels.
 // wf_extend    per queued path: Russian roulette (program option), the
-//              nearest-hit scan, the sky on a miss. A hit's record goes to
-//              the SoA hit store and the path to one of WF_KEYS shade queues
-//              by the material component its next interaction will take
-//              (the mixture's choice, read from a copy of the path's RNG, so
-//              the interaction itself draws the same uniform later).
+//              nearest-hit scan, the sky on a miss. A hit moves the path,
+//              with its hit record, into one of WF_KEYS shade queues by the
+//              material component its next interaction will take (the
+//              mixture's choice, read from a copy of the path's RNG, so the
+//              interaction itself draws the same uniform later).
 // wf_shade     the shade queues in key order (a wave runs one material): the
-//              material interaction; a path that goes on re-enters the
-//              extend queue, one that ends adds its radiance to the pixel and
+//              material interaction; a path that goes on moves to the next
+//              extend queue, one that ends adds its radiance to its pixel and
 //              starts the pixel's next sample (regeneration) until `passes`.
 //
-// Path state and hit records live in HBM as SoA: word k of path p's record
-// at base[k * cap + p], cap = w * h, so every load and store of a wave is
-// coalesced. The state is two records: the ray part (RNG, hasher, light),
-// which the scan needs, and the accumulator part (emission, hop, samples
-// done), which extend touches only when the path ends there. Queues are
-// compacted per wavefront (ballot + lane prefix count, one atomic per wave
-// and queue). A pixel has one live path at a time, so
-// the queues never exceed cap and its canvas word has one writer.
+// SoA queues: a queue holds the records of its paths, word k of the record
+// in slot i at base[k * cap + i] (cap = w * h; a pixel has one live path at
+// a time, so no queue holds more). A kernel reads its input queue's slots in
+// order and writes each surviving path's record into the slot the output
+// queue assigns it; slots are handed out per wavefront (ballot + lane
+// prefix count, one atomic per wave and queue), so a wave's 64 records land
+// in consecutive slots and every load and store is coalesced. Extend-queue
+// records: the ray part (RNG, hasher, light) and the accumulator part
+// (emission, hop, samples done, pixel); shade-queue records add the hit.
 //
 // Counters (ctr): extend-queue sizes [WF_CTR_EXT + parity] and shade-queue
 // sizes [WF_CTR_SH + parity * WF_KEYS + key], double-buffered by iteration
@@ -66,8 +67,29 @@ template <class S> struct WfRay {
 };
 struct WfAcc {
     color3 emission;
-    int hop;    // hits of the current path so far
-    uint done;  // finished samples of the pixel
+    int hop;     // hits of the current path so far
+    uint done;   // finished samples of the pixel
+    uint pixel;  // index into canvas and seeds
+};
+
+// The records of one queue: ray part, accumulator part and (shade queues)
+// the hit, one after the other, each `cap` records of its word count.
+template <class S> struct WfQueue {
+    typedef typename S::Object::Cache Hit;
+    static constexpr uint RAY = Soa<WfRay<S>>::WORDS, ACC = Soa<WfAcc>::WORDS, HIT = Soa<Hit>::WORDS;
+    static constexpr uint EXT_WORDS = RAY + ACC, SHADE_WORDS = RAY + ACC + HIT;
+    static HD WfRay<S> ray(const uint *q, uint cap, uint i) { return Soa<WfRay<S>>::load(q, cap, i); }
+    static HD WfAcc acc(const uint *q, uint cap, uint i) { return Soa<WfAcc>::load(q + (size_t)RAY * cap, cap, i); }
+    static HD Hit hit(const uint *q, uint cap, uint i) {
+        return Soa<Hit>::load(q + (size_t)EXT_WORDS * cap, cap, i);
+    }
+    static HD void put(uint *q, uint cap, uint i, const WfRay<S> &r, const WfAcc &a) {
+        Soa<WfRay<S>>::store(q, cap, i, r);
+        Soa<WfAcc>::store(q + (size_t)RAY * cap, cap, i, a);
+    }
+    static HD void put_hit(uint *q, uint cap, uint i, const Hit &h) {
+        Soa<Hit>::store(q + (size_t)EXT_WORDS * cap, cap, i, h);
+    }
 };
 
 // ------------------------------------------------------------ material keys
@@ -121,27 +143,29 @@ template <class G, class E> struct HitKey<ObjectVec<G, E>> {
 };
 
 // ------------------------------------------------------------ helpers
-// Appends `value` for the lanes that `want`: one atomic per wave. Called by
-// every lane of the wave (wave-uniform control flow). A queue holds at most
-// one entry per pixel, so slots stay below `cap`; the bound is also checked.
-HD void wf_push(bool want, uint *__restrict__ counter, uint *__restrict__ queue, uint value, uint cap) {
+// Slot for this lane's record (if `want`) in a queue: one atomic per wave.
+// Called by every lane of the wave (wave-uniform control flow). Returns ~0u
+// when the lane has no record (or, which the one-path-per-pixel rule
+// excludes, the queue is full).
+HD uint wf_slot(bool want, uint *__restrict__ counter, uint cap) {
     unsigned long long m = __ballot(want);
-    if (m == 0) return;
+    if (m == 0) return ~0u;
     uint leader = (uint)__ffsll((long long)m) - 1u;
     uint base = 0;
     if (__lane_id() == leader) base = atomicAdd(counter, (uint)__popcll(m));
     base = __shfl(base, (int)leader);
     uint slot = base + (uint)__popcll(m & lane_mask_below());
-    if (want && slot < cap) queue[slot] = value;
+    return want && slot < cap ? slot : ~0u;
 }
 
-// The path of pixel p has ended: its radiance joins the pixel (the canvas
-// word holds the running sum, the order of render_pixel_passes' additions);
-// the pixel's next sample starts (returns true) until `passes` are done,
-// then its RNG state goes back to `seeds`.
+// The path has ended: its radiance joins its pixel (the canvas word holds
+// the running sum, the order of render_pixel_passes' additions); the pixel's
+// next sample starts (returns true) until `passes` are done, then its RNG
+// state goes back to `seeds`.
 template <class S>
-HD bool wf_finish(uint w, uint h, uint p, const u8 *scene, float4 *__restrict__ canvas, uint *__restrict__ seeds,
+HD bool wf_finish(uint w, uint h, const u8 *scene, float4 *__restrict__ canvas, uint *__restrict__ seeds,
                   uint passes, WfRay<S> *r, WfAcc *a) {
+    uint p = a->pixel;
     float4 acc = canvas[p];
     acc.x += a->emission.x;
     acc.y += a->emission.y;
@@ -162,8 +186,7 @@ HD bool wf_finish(uint w, uint h, uint p, const u8 *scene, float4 *__restrict__ 
 // ------------------------------------------------------------ kernels
 template <class S>
 HD void wf_generate_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
-                         uint *__restrict__ seeds, uint passes, uint *__restrict__ rays, uint *__restrict__ accs,
-                         uint *__restrict__ q0, uint *__restrict__ ctr) {
+                         uint *__restrict__ seeds, uint passes, uint *__restrict__ q0, uint *__restrict__ ctr) {
     uint n = w * h;
     uint idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx == 0) {
@@ -189,19 +212,17 @@ HD void wf_generate_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *_
         start_path<S>(scene, &r.ctx, px, &r.light, &a.emission);
         a.hop = 0;
         a.done = 0u;
-        Soa<WfRay<S>>::store(rays, n, idx, r);
-        Soa<WfAcc>::store(accs, n, idx, a);
-        q0[idx] = idx;
+        a.pixel = idx;
+        WfQueue<S>::put(q0, n, idx, r, a);
     }
 }
 
 template <class S>
 HD void wf_extend_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
-                       uint *__restrict__ seeds, uint passes, uint *__restrict__ rays, uint *__restrict__ accs,
-                       uint *__restrict__ hits,
-                       const uint *__restrict__ q_in, uint *__restrict__ q_next, uint *__restrict__ q_shade,
-                       uint *__restrict__ ctr, uint parity) {
+                       uint *__restrict__ seeds, uint passes, const uint *__restrict__ q_in,
+                       uint *__restrict__ q_next, uint *__restrict__ q_shade, uint *__restrict__ ctr, uint parity) {
     typedef typename S::Object T;
+    typedef WfQueue<S> Q;
     const u8 *obj = S::object(scene);
     uint cap = w * h;
     uint n = min(ctr[WF_CTR_EXT + parity], cap);
@@ -211,49 +232,53 @@ HD void wf_extend_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__r
     uint stride = gridDim.x * blockDim.x;
     for (uint i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < n; i0 += stride) {
         uint i = i0 + __lane_id();
-        uint p = 0u, key = WF_KEYS;
+        uint key = WF_KEYS;
         bool next = false;
-        if (i < n) p = q_in[i];
-        if (i < n && p < cap) {
-            WfRay<S> r = Soa<WfRay<S>>::load(rays, cap, p);
+        WfRay<S> r;
+        WfAcc a;
+        typename T::Cache cache;
+        if (i < n) {
+            r = Q::ray(q_in, cap, i);
+            a = Q::acc(q_in, cap, i);
             bool end = false, miss = false;
 #ifdef HT_RR_DEPTH
-            if (Soa<WfAcc>::load(accs, cap, p).hop >= HT_RR_DEPTH && russian_roulette(&r.ctx, &r.light.base))
-                end = true;
+            if (a.hop >= HT_RR_DEPTH && russian_roulette(&r.ctx, &r.light.base)) end = true;
 #endif
             if (!end) {
                 r.ctx.hasher = hash_init();
-                typename T::Cache cache;
                 real dist = TopScan<T>::detect(obj, &r.ctx, &cache, &r.light);
                 if (dist > (real)-0.5f) {
                     uint k = HitKey<T>::key(obj, &cache, r.ctx.rng);
                     key = k < WF_KEYS ? k : WF_KEYS - 1u;
-                    Soa<typename T::Cache>::store(hits, cap, p, cache);
                 } else {
                     miss = end = true;
                 }
             }
             if (end) {
-                WfAcc a = Soa<WfAcc>::load(accs, cap, p);
                 if (miss) S::bg(scene, &r.ctx, &r.light, &a.emission);
-                next = wf_finish<S>(w, h, p, scene, canvas, seeds, passes, &r, &a);
-                Soa<WfAcc>::store(accs, cap, p, a);
+                next = wf_finish<S>(w, h, scene, canvas, seeds, passes, &r, &a);
             }
-            Soa<WfRay<S>>::store(rays, cap, p, r);
         }
-        wf_push(next, ctr + WF_CTR_EXT + (parity ^ 1u), q_next, p, cap);
+        uint slot = wf_slot(next, ctr + WF_CTR_EXT + (parity ^ 1u), cap);
+        if (slot != ~0u) Q::put(q_next, cap, slot, r, a);
 #pragma unroll
-        for (uint k = 0; k < WF_KEYS; ++k)
-            wf_push(key == k, ctr + WF_CTR_SH + parity * WF_KEYS + k, q_shade + (size_t)k * cap, p, cap);
+        for (uint k = 0; k < WF_KEYS; ++k) {
+            uint s = wf_slot(key == k, ctr + WF_CTR_SH + parity * WF_KEYS + k, cap);
+            if (s != ~0u) {
+                uint *q = q_shade + (size_t)k * Q::SHADE_WORDS * cap;
+                Q::put(q, cap, s, r, a);
+                Q::put_hit(q, cap, s, cache);
+            }
+        }
     }
 }
 
 template <class S>
 HD void wf_shade_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
-                      uint *__restrict__ seeds, uint passes, uint *__restrict__ rays, uint *__restrict__ accs,
-                      const uint *__restrict__ hits, const uint *__restrict__ q_shade, uint *__restrict__ q_next,
-                      uint *__restrict__ ctr, uint parity) {
+                      uint *__restrict__ seeds, uint passes, const uint *__restrict__ q_shade,
+                      uint *__restrict__ q_next, uint *__restrict__ ctr, uint parity) {
     typedef typename S::Object T;
+    typedef WfQueue<S> Q;
     const u8 *obj = S::object(scene);
     uint cap = w * h;
     uint counts[WF_KEYS], total = 0u;
@@ -271,8 +296,9 @@ HD void wf_shade_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__re
     uint stride = gridDim.x * blockDim.x;
     for (uint i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); i0 < total; i0 += stride) {
         uint i = i0 + __lane_id();
-        uint p = 0u;
         bool next = false;
+        WfRay<S> r;
+        WfAcc a;
         if (i < total) {
             // entry i of the key-ordered concatenation of the shade queues
             uint k = 0u, j = i;
@@ -280,52 +306,44 @@ HD void wf_shade_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__re
                 j -= counts[k];
                 ++k;
             }
-            p = q_shade[(size_t)k * cap + j];
-        }
-        if (i < total && p < cap) {
-            WfRay<S> r = Soa<WfRay<S>>::load(rays, cap, p);
-            WfAcc a = Soa<WfAcc>::load(accs, cap, p);
-            typename T::Cache cache = Soa<typename T::Cache>::load(hits, cap, p);
+            const uint *q = q_shade + (size_t)k * Q::SHADE_WORDS * cap;
+            r = Q::ray(q, cap, j);
+            a = Q::acc(q, cap, j);
+            typename T::Cache cache = Q::hit(q, cap, j);
             bool end = !T::interact(obj, &r.ctx, &cache, &r.light, &a.emission);
             if (!end) r.ctx.prev_hash = hash_finish(&r.ctx.hasher);
             a.hop += 1;
             if (a.hop >= S::HOPS) end = true;
-            next = end ? wf_finish<S>(w, h, p, scene, canvas, seeds, passes, &r, &a) : true;
-            Soa<WfRay<S>>::store(rays, cap, p, r);
-            Soa<WfAcc>::store(accs, cap, p, a);
+            next = end ? wf_finish<S>(w, h, scene, canvas, seeds, passes, &r, &a) : true;
         }
-        wf_push(next, ctr + WF_CTR_EXT + (parity ^ 1u), q_next, p, cap);
+        uint slot = wf_slot(next, ctr + WF_CTR_EXT + (parity ^ 1u), cap);
+        if (slot != ~0u) Q::put(q_next, cap, slot, r, a);
     }
 }
 
 }  // namespace ht
 
 // Kernel entry points of the wavefront form for the scene type `S`, and the
-// record sizes the runtime allocates the SoA stores by: {ray words,
-// accumulator words, hit words, shade keys, counter words, HOPS}.
+// queue sizes the runtime allocates by: {extend-record words, shade-record
+// words, shade keys, counter words, HOPS}.
 #define HT_DEFINE_WAVEFRONT_KERNELS(S)                                                                          \
-    extern "C" __device__ unsigned ht_wf_layout[6] = {                                                          \
-        ht::Soa<ht::WfRay<S>>::WORDS, ht::Soa<ht::WfAcc>::WORDS, ht::Soa<typename S::Object::Cache>::WORDS,     \
-        ht::WF_KEYS, ht::WF_CTR_WORDS, (unsigned)S::HOPS};                                                      \
+    extern "C" __device__ unsigned ht_wf_layout[5] = {ht::WfQueue<S>::EXT_WORDS, ht::WfQueue<S>::SHADE_WORDS,   \
+                                                      ht::WF_KEYS, ht::WF_CTR_WORDS, (unsigned)S::HOPS};        \
     extern "C" __global__ void __launch_bounds__(256) wf_generate(                                              \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas,           \
-        unsigned *__restrict__ seeds, unsigned passes, unsigned *__restrict__ rays, unsigned *__restrict__ accs, \
-        unsigned *__restrict__ q0, unsigned *__restrict__ ctr) {                                                \
-        ht::wf_generate_impl<S>(w, h, scene, canvas, seeds, passes, rays, accs, q0, ctr);                       \
+        unsigned *__restrict__ seeds, unsigned passes, unsigned *__restrict__ q0, unsigned *__restrict__ ctr) { \
+        ht::wf_generate_impl<S>(w, h, scene, canvas, seeds, passes, q0, ctr);                                   \
     }                                                                                                           \
     extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU) wf_extend(                               \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas,           \
-        unsigned *__restrict__ seeds, unsigned passes, unsigned *__restrict__ rays, unsigned *__restrict__ accs, \
-        unsigned *__restrict__ hits, const unsigned *__restrict__ q_in, unsigned *__restrict__ q_next,          \
-        unsigned *__restrict__ q_shade, unsigned *__restrict__ ctr, unsigned parity) {                          \
-        ht::wf_extend_impl<S>(w, h, scene, canvas, seeds, passes, rays, accs, hits, q_in, q_next, q_shade, ctr, \
-                              parity);                                                                          \
+        unsigned *__restrict__ seeds, unsigned passes, const unsigned *__restrict__ q_in,                       \
+        unsigned *__restrict__ q_next, unsigned *__restrict__ q_shade, unsigned *__restrict__ ctr,              \
+        unsigned parity) {                                                                                      \
+        ht::wf_extend_impl<S>(w, h, scene, canvas, seeds, passes, q_in, q_next, q_shade, ctr, parity);          \
     }                                                                                                           \
     extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU) wf_shade(                                \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas,           \
-        unsigned *__restrict__ seeds, unsigned passes, unsigned *__restrict__ rays, unsigned *__restrict__ accs, \
-        const unsigned *__restrict__ hits, const unsigned *__restrict__ q_shade, unsigned *__restrict__ q_next, \
-        unsigned *__restrict__ ctr, unsigned parity) {                                                          \
-        ht::wf_shade_impl<S>(w, h, scene, canvas, seeds, passes, rays, accs, hits, q_shade, q_next, ctr,        \
-                             parity);                                                                           \
+        unsigned *__restrict__ seeds, unsigned passes, const unsigned *__restrict__ q_shade,                    \
+        unsigned *__restrict__ q_next, unsigned *__restrict__ ctr, unsigned parity) {                           \
+        ht::wf_shade_impl<S>(w, h, scene, canvas, seeds, passes, q_shade, q_next, ctr, parity);                 \
     }

@@ -45,10 +45,10 @@ struct ht_prog {
     hipFunction_t render_batch = nullptr;   // batch programs (several scene types, one persistent kernel)
     hipFunction_t bounds_fill = nullptr;    // scalar-bounds programs: fills the bounds before each render launch
     // Wavefront form (programs whose root source adds the wavefront kernels):
-    // kernels, record sizes {ray words, accumulator words, hit words, keys,
-    // counter words, HOPS} and the SoA / queue workspace, grown on demand.
+    // kernels, queue sizes {extend-record words, shade-record words, keys,
+    // counter words, HOPS} and the queue workspace, grown on demand.
     hipFunction_t wf_generate = nullptr, wf_extend = nullptr, wf_shade = nullptr;
-    unsigned wf_layout[6] = {0, 0, 0, 0, 0, 0};
+    unsigned wf_layout[5] = {0, 0, 0, 0, 0};
     void *wf_mem = nullptr;
     size_t wf_bytes = 0;
     // Held from a scalar-bounds program's bounds_fill enqueue to its render

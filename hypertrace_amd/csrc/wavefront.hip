@@ -41,12 +41,11 @@ extern "C" int ht_render_wavefront(ht_prog *prog, uint32_t w, uint32_t h, ht_buf
     if ((rc = check_size("ht_render_wavefront canvas", canvas, (size_t)n * 16))) return rc;
     if ((rc = check_size("ht_render_wavefront seeds", seeds, (size_t)n * 4))) return rc;
     if (scene->size == 0) return fail(HT_ERR_OTHER, "ht_render_wavefront: empty scene buffer");
-    const uint32_t ray_words = prog->wf_layout[0], acc_words = prog->wf_layout[1], hit_words = prog->wf_layout[2];
-    const uint32_t keys = prog->wf_layout[3], ctr_words = prog->wf_layout[4], hops = prog->wf_layout[5];
-    // workspace: ray and accumulator state, hit records, 2 extend queues,
-    // `keys` shade queues (n words each per record word / queue), then the
-    // counters
-    const size_t words = (size_t)n * (ray_words + acc_words + hit_words + 2 + keys) + ctr_words;
+    const uint32_t ext_words = prog->wf_layout[0], shade_words = prog->wf_layout[1], keys = prog->wf_layout[2];
+    const uint32_t ctr_words = prog->wf_layout[3], hops = prog->wf_layout[4];
+    // workspace: 2 extend queues and `keys` shade queues of n SoA records
+    // each, then the counters
+    const size_t words = (size_t)n * (2 * (size_t)ext_words + (size_t)keys * shade_words) + ctr_words;
     const size_t bytes = words * 4;
     ht_ctx *ctx = prog->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
@@ -62,17 +61,14 @@ extern "C" int ht_render_wavefront(ht_prog *prog, uint32_t w, uint32_t h, ht_buf
         prog->wf_bytes = bytes;
     }
     uint32_t *base = static_cast<uint32_t *>(prog->wf_mem);
-    uint32_t *rays = base;
-    uint32_t *accs = rays + (size_t)n * ray_words;
-    uint32_t *hits = accs + (size_t)n * acc_words;
-    uint32_t *q_ext[2] = {hits + (size_t)n * hit_words, hits + (size_t)n * (hit_words + 1)};
-    uint32_t *q_shade = q_ext[1] + n;
-    uint32_t *ctr = q_shade + (size_t)n * keys;
+    uint32_t *q_ext[2] = {base, base + (size_t)n * ext_words};
+    uint32_t *q_shade = q_ext[1] + (size_t)n * ext_words;
+    uint32_t *ctr = q_shade + (size_t)n * keys * shade_words;
 
     void *sp = scene->ptr, *cp = canvas->ptr, *pp = seeds->ptr;
     if ((rc = ht_internal::fill_bounds(prog, sp))) return rc;
     {
-        void *args[] = {&w, &h, &sp, &cp, &pp, &passes, &rays, &accs, &q_ext[0], &ctr};
+        void *args[] = {&w, &h, &sp, &cp, &pp, &passes, &q_ext[0], &ctr};
         HIP_TRY(hipModuleLaunchKernel(prog->wf_generate, (n + 255) / 256, 1, 1, 256, 1, 1, 0, ctx->stream, args,
                                       nullptr));
     }
@@ -86,12 +82,10 @@ extern "C" int ht_render_wavefront(ht_prog *prog, uint32_t w, uint32_t h, ht_buf
         for (uint32_t c = 0; c < WF_CHUNK; ++c, ++iter) {
             uint32_t parity = (uint32_t)(iter & 1);
             uint32_t *q_in = q_ext[parity], *q_next = q_ext[parity ^ 1];
-            void *ext_args[] = {&w, &h, &sp, &cp, &pp, &passes, &rays, &accs, &hits, &q_in, &q_next, &q_shade,
-                                &ctr, &parity};
+            void *ext_args[] = {&w, &h, &sp, &cp, &pp, &passes, &q_in, &q_next, &q_shade, &ctr, &parity};
             HIP_TRY(hipModuleLaunchKernel(prog->wf_extend, blocks, 1, 1, 256, 1, 1, 0, ctx->stream, ext_args,
                                           nullptr));
-            void *sh_args[] = {&w, &h, &sp, &cp, &pp, &passes, &rays, &accs, &hits, &q_shade, &q_next, &ctr,
-                               &parity};
+            void *sh_args[] = {&w, &h, &sp, &cp, &pp, &passes, &q_shade, &q_next, &ctr, &parity};
             HIP_TRY(hipModuleLaunchKernel(prog->wf_shade, blocks, 1, 1, 256, 1, 1, 0, ctx->stream, sh_args,
                                           nullptr));
         }

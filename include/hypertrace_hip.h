@@ -101,14 +101,14 @@ int ht_render_passes(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_bu
                      ht_buf *seeds_u32, uint32_t passes);
 
 /* The same result as ht_render_passes (canvas and seeds bit for bit), as a
- * wavefront: paths live in SoA state in device memory and advance one hit
+ * wavefront: paths live in SoA queues in device memory and advance one hit
  * per iteration through an extend kernel (nearest hit; hits sorted into
  * queues by the material component they take next) and a shade kernel
  * (material step in queue order, regeneration of the pixel's next sample).
  * The program's root source must add HT_DEFINE_WAVEFRONT_KERNELS(Scene)
- * (codegen.root_source(..., wavefront=True)); its workspace (4 bytes x
- * (path-state + hit-record words + 6) per pixel, ~170 B for the cell scene)
- * is kept by the program and grown on demand. Synchronous. No reference counterpart: the form the
+ * (codegen.root_source(..., wavefront=True)); its workspace (2 extend and 4
+ * shade queues of one record per pixel, ~550 B per pixel for the cell
+ * scene) is kept by the program and grown on demand. Synchronous. No reference counterpart: the form the
  * megakernel is measured against. */
 int ht_render_wavefront(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas_f32x4,
                         ht_buf *seeds_u32, uint32_t passes);
