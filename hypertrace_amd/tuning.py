@@ -22,7 +22,7 @@ from .processing import STRICT_FP, Canvas, Context, EntityBuffer, Render
 
 # (cull unroll, waves per SIMD, scalar bounds)
 CANDIDATES: Tuple[Tuple[int, int, bool], ...] = ((4, 4, False), (4, 5, False), (2, 5, False), (4, 6, False),
-                                                 (4, 4, True), (4, 5, True))
+                                                 (4, 4, True), (4, 5, True), (4, 6, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False) -> str:
