@@ -247,13 +247,15 @@ def cache_leg(args):
 
 
 def fixed_knobs(text, base=None):
-    """A render-tuner result for the setting "UNROLL,WAVES[,sb]" (no timing)."""
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN]" (no timing)."""
     from hypertrace_amd.processing import STRICT_FP
     from hypertrace_amd.tuning import options
     knobs = text.split(",")
-    unroll, waves, sb = int(knobs[0]), int(knobs[1]), knobs[2:] == ["sb"]
-    return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb,
-                     "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb)},
+    unroll, waves = int(knobs[0]), int(knobs[1])
+    sb = "sb" in knobs[2:]
+    groups = next((int(k[1:]) for k in knobs[2:] if k.startswith("g")), 0)
+    return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
+                     "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups)},
             "table": [], "fixed": True}
 
 
@@ -425,10 +427,10 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
-    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb]",
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN]",
                     help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
                          "instead of tuning (keeps a profiled trace to one program)")
-    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb]",
+    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN]",
                     help="the same for the SDF leg's program")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "

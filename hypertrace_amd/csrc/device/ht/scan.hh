@@ -51,9 +51,17 @@ static constexpr int CULL_UNROLL = HT_CULL_UNROLL;
 #define HT_CULL_FP _Pragma("clang fp contract(fast)")
 static constexpr int WAVE = 64;
 // Pair-queue ring of the culled scan: holds the pairs left after a batch
-// (< 64) plus one step's appends (<= 64 x CULL_UNROLL). Every index formed
-// is below twice its size, so one conditional subtraction wraps it.
-static constexpr uint QUEUE_RING = WAVE * (HT_CULL_UNROLL + 1);
+// (< 64) plus the appends between two drains (<= 64 per element: the
+// CULL_UNROLL elements of a flat step, or the leaves of a group node with
+// HT_CULL_GROUPS). Every index formed is below twice its size, so one
+// conditional subtraction wraps it.
+#if defined(HT_CULL_GROUPS) && HT_CULL_GROUPS > HT_CULL_UNROLL
+#define HT_RING_STEP HT_CULL_GROUPS
+#else
+#define HT_RING_STEP HT_CULL_UNROLL
+#endif
+static constexpr uint RING_STEP = HT_RING_STEP;
+static constexpr uint QUEUE_RING = WAVE * (RING_STEP + 1);
 HD uint ring_wrap(uint i) { return i >= QUEUE_RING ? i - QUEUE_RING : i; }
 #ifndef HT_MAX_WAVES
 #define HT_MAX_WAVES 4  // blocks of up to 256 threads
@@ -436,10 +444,20 @@ template <class G> HD WaveScratch<G> *wave_scratch_lds() {
 // is a dependent chain (node load -> FP64 test -> next node) at low
 // occupancy, and nodes holding a horoball cannot use the "behind" cull. The
 // flat scan is the default.
-#if defined(HT_CULL_TREE) && defined(HT_SCALAR_BOUNDS)
-#error "HT_CULL_TREE walks the LDS bounds; HT_SCALAR_BOUNDS has none"
+#if (defined(HT_CULL_TREE) || defined(HT_CULL_GROUPS)) && defined(HT_SCALAR_BOUNDS)
+#error "HT_CULL_TREE / HT_CULL_GROUPS walk the LDS bounds; HT_SCALAR_BOUNDS has none"
 #endif
-#ifdef HT_CULL_TREE
+// Program option HT_CULL_GROUPS = g (2..16, a power of two): the same tree,
+// scanned in two levels with every lane of a wave in step: the nodes that
+// cover g consecutive leaves are tested first, and a node's leaves only when
+// some lane's ray passes it (a wave-uniform skip).
+#ifdef HT_CULL_GROUPS
+static constexpr uint CULL_GROUP = HT_CULL_GROUPS;
+static_assert(CULL_GROUP >= 2 && CULL_GROUP <= 16 && (CULL_GROUP & (CULL_GROUP - 1)) == 0, "group size");
+static_assert(CULL_GROUP <= RING_STEP, "the pair ring holds one group's leaves");
+#endif
+static_assert((uint)CULL_UNROLL <= RING_STEP, "the pair ring holds one flat step");
+#if defined(HT_CULL_TREE) || defined(HT_CULL_GROUPS)
 static constexpr bool TREE_OK = BOUNDS_CAP >= 64 && BOUNDS_CAP <= 1024;
 #else
 static constexpr bool TREE_OK = false;
@@ -769,6 +787,34 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         }
     }
 
+#ifdef HT_CULL_GROUPS
+    // The group nodes of the tree in order, each tested by every lane; a
+    // node's CULL_GROUP leaves are tested (and their pairs queued) only when
+    // some lane's ray passes the node. HT_CULL_GROUPS only.
+    template <class CR, class Push, class Drain>
+    static HD void walk_groups(const CR &cr, Push &push, Drain &drain) {
+        constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP, FIRST = P / GS;
+        const Bound *bounds = bounds_lds<OV>();
+        const TreeLds *t = tree_lds<OV>();
+        for (uint k = FIRST; k < 2 * FIRST; ++k) {
+            float4 q = t->node[k];
+            if (q.w == -2.f) continue;  // no elements under this node
+            bool pass = cr.candidate(MakeBound<Bound>::node(q, t->horo[k] != 0));
+            if (__ballot(pass) == 0) continue;
+            bool c[GS];
+            unsigned short el[GS];
+#pragma unroll
+            for (uint j = 0; j < GS; ++j) {
+                el[j] = t->order[k * GS - P + j];
+                c[j] = el[j] != NO_ELEMENT && cr.candidate(bounds[el[j] != NO_ELEMENT ? el[j] : 0]) && pass;
+            }
+#pragma unroll
+            for (uint j = 0; j < GS; ++j) push(el[j], c[j]);
+            drain();
+        }
+    }
+#endif
+
     static HD real detect(const u8 *self, Context *ctx, typename OV::Cache *cache, Light<G> *light) {
         if constexpr (!CULL) {
             return OV::detect(self, ctx, cache, light);
@@ -823,7 +869,11 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             bool tree = false;
             if constexpr (TREE_OK) tree = n >= TREE_MIN_ELEMENTS;
             if (tree) {
+#ifdef HT_CULL_GROUPS
+                if constexpr (TREE_OK) walk_groups(cr, push, drain);
+#else
                 if constexpr (TREE_OK) walk_tree(cr, append);
+#endif
             } else {
                 // CULL_UNROLL independent cull tests per step (instruction-level
                 // parallelism, traded against registers and so occupancy).

@@ -1,13 +1,14 @@
 """Autotuner of the render program's code-generation knobs for a workload.
 
-Three compile-time knobs of the scene program: HT_CULL_UNROLL (cull tests
+Four compile-time knobs of the scene program: HT_CULL_UNROLL (cull tests
 per step of the nearest-hit scan) and HT_WAVES_PER_EU (minimum waves per
 SIMD the register allocation must allow; above 4 the compiler spills a
 little) trade instruction-level parallelism against registers and so
 against resident waves; HT_SCALAR_BOUNDS moves the element bounds from LDS
 (FP32, widened per lane) to a global FP64 array read through the scalar
 cache, which frees the LDS that limits occupancy on scenes with many
-elements. Every setting renders bit-identical images (only the schedule
+elements; HT_CULL_GROUPS scans groups of Morton-ordered elements first and
+skips a group's elements when no lane of the wave passes its sphere. Every setting renders bit-identical images (only the schedule
 differs), so the tuner simply times one frame of the caller's workload per
 candidate on the GPU and keeps the fastest. Small frames (few waves per
 SIMD) prefer fewer registers less than large ones.
@@ -20,26 +21,33 @@ from typing import Dict, Optional, Sequence, Tuple
 
 from .processing import STRICT_FP, Canvas, Context, EntityBuffer, Render
 
-# (cull unroll, waves per SIMD, scalar bounds)
-CANDIDATES: Tuple[Tuple[int, int, bool], ...] = ((4, 4, False), (4, 5, False), (2, 5, False), (4, 6, False),
-                                                 (4, 4, True), (4, 5, True), (4, 6, True))
+# (cull unroll, waves per SIMD, scalar bounds, group size or 0)
+CANDIDATES: Tuple[Tuple[int, int, bool, int], ...] = (
+    (4, 4, False, 0), (4, 5, False, 0), (2, 5, False, 0), (4, 6, False, 0),
+    (4, 4, True, 0), (4, 5, True, 0), (4, 6, True, 0), (4, 5, False, 4))
 
 
-def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False) -> str:
+def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
-    return opts + " -DHT_SCALAR_BOUNDS=1" if scalar_bounds else opts
+    if scalar_bounds:
+        opts += " -DHT_SCALAR_BOUNDS=1"
+    if groups:
+        opts += f" -DHT_CULL_GROUPS={int(groups)}"
+    return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int]:
     unroll, waves = int(candidate[0]), int(candidate[1])
-    return unroll, waves, bool(candidate[2]) if len(candidate) > 2 else False
+    sb = bool(candidate[2]) if len(candidate) > 2 else False
+    groups = int(candidate[3]) if len(candidate) > 3 else 0
+    return unroll, waves, sb, groups
 
 
 def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mode: int = 1, seed: int = 0,
                 candidates: Sequence[Sequence] = CANDIDATES, reps: int = 3, base: str = STRICT_FP,
                 cache_path: Optional[str] = None) -> Dict:
     """Times `spp` passes of render_passes (HIP events, median of `reps`) for
-    each (unroll, waves[, scalar bounds]) candidate; returns {"best": {...},
+    each (unroll, waves[, scalar bounds[, groups]]) candidate; returns {"best": {...},
     "table": [...]} with the program options of the best."""
     key = f"gfx950:{scene.cpp()}:{width}x{height}:{spp}:{base}"
     if cache_path and os.path.exists(cache_path):
@@ -50,8 +58,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb = _knobs(cand)
-        opts = options(unroll, waves, base, sb)
+        unroll, waves, sb, groups = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -61,7 +69,7 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             ctx.timer_begin()
             r.render_passes(buf, c, spp)
             times.append(ctx.timer_end())
-        table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb,
+        table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
                       "ms": sorted(times)[len(times) // 2], "options": opts})
     best = min(table, key=lambda t: t["ms"])
     result = {"best": best, "table": table}

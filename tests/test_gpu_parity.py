@@ -190,6 +190,8 @@ CULLED_VARIANTS = {
     "unroll2_waves5": STRICT_FP + " -DHT_CULL_UNROLL=2 -DHT_WAVES_PER_EU=5",
     "scalar_bounds": STRICT_FP + " -DHT_SCALAR_BOUNDS=1",
     "scalar_bounds_u2_w5": STRICT_FP + " -DHT_CULL_UNROLL=2 -DHT_WAVES_PER_EU=5 -DHT_SCALAR_BOUNDS=1",
+    "groups4": STRICT_FP + " -DHT_CULL_GROUPS=4",
+    "groups8_w5": STRICT_FP + " -DHT_CULL_GROUPS=8 -DHT_WAVES_PER_EU=5",
 }
 
 
@@ -199,7 +201,8 @@ CULLED_VARIANTS = {
     ("sdf_1024", False, "flat"), ("hy_cells", False, "tree"), ("sdf_1024", False, "tree"),
     ("hy_cells", True, "unroll2_waves5"), ("eu_ball", False, "scalar_bounds"), ("hy_cells", True, "scalar_bounds"),
     ("hy_example", False, "scalar_bounds"), ("sdf_1024", False, "scalar_bounds"),
-    ("hy_cells", False, "scalar_bounds_u2_w5")])
+    ("hy_cells", False, "scalar_bounds_u2_w5"), ("hy_cells", True, "groups4"), ("sdf_1024", False, "groups4"),
+    ("hy_cells", False, "groups8_w5")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
