@@ -130,17 +130,13 @@ __device__ BoundS ht_scalar_bounds[HT_BOUNDS_CAP];
 // same memory; no kernel that reads it writes it), which lets wave-uniform
 // loads go through the scalar cache into SGPRs.
 typedef __attribute__((address_space(4))) const BoundS ConstBoundS;
+// One 64-byte scalar load per bound (s_load_dwordx16), whatever fields the
+// test uses.
+typedef unsigned int u32x16 __attribute__((ext_vector_type(16)));
 HD BoundS load_bound(const ConstBoundS *p) {
+    u32x16 v = *reinterpret_cast<__attribute__((address_space(4))) const u32x16 *>(p);
     BoundS b;
-    b.cx = p->cx;
-    b.cy = p->cy;
-    b.r2 = p->r2;
-    b.ri2 = p->ri2;
-    b.cz2 = p->cz2;
-    b.cz = p->cz;
-    b.r = p->r;
-    b.horo = p->horo;
-    b.pad = 0.0f;
+    __builtin_memcpy(&b, &v, sizeof(b));
     return b;
 }
 #endif
