@@ -212,8 +212,9 @@ def cache_leg(args):
         plain = None if args.no_plain else Render(ctx, wl.scene, fp_options=NO_CULL)
         base = time_to_rmse(ctx, wl, buf, chosen, plain, args, reference)
         cache = RadianceCache(ctx, W * H, 65536, 16384)
+        # the cached kernel runs the same scan: build it with the tuner's options
         cr = CachedRender(ctx, wl.scene, cache, query_hop=2, train_every=16, steps_per_pass=4, batch=16384,
-                          lr=1e-3)
+                          lr=1e-3, fp_options=tuning["best"]["options"])
         canvas = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
         t, spp, curve, reached, check = 0.0, 0, [], None, 4
         while spp < args.cache_max_spp:
