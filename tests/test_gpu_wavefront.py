@@ -72,3 +72,23 @@ def ctx():
     with Context(0, X32_F32) as c:
         yield c
 
+
+
+def test_wavefront_fp64_program():
+    """FP64 programs (DOUBLE_PRECISION: real = double, FP64 hit keys) through
+    the wavefront form give the same canvas and seeds as render_passes."""
+    from hypertrace_amd.types import X32_F64
+
+    wl = scenes.hyperbolic_cells(80, 60)
+    with Context(0, X32_F64) as c64:
+        buf = EntityBuffer(c64, wl.scene)
+        mega = Render(c64, wl.scene)
+        wave = Render(c64, wl.scene, wavefront=True)
+        out = []
+        for run in (lambda c: mega.render_passes(buf, c, 4), lambda c: wave.render_wavefront(buf, c, 4)):
+            c = Canvas(c64, (80, 60), wl.seed_mode, wl.seed)
+            run(c)
+            c64.sync()
+            out.append((c.load_image().copy(), c.load_seeds().copy()))
+    np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
