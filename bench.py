@@ -378,7 +378,7 @@ def batch_leg(args):
     with Context(0, X32_F32) as ctx:
         tune = autotune(ctx, specs, reps=1)
         best = tune["best"]
-        bprog = BatchProgram(ctx, best["block"])
+        bprog = BatchProgram(ctx, best["block"], fp_options=best.get("options"))
         batch = Batch(ctx, bprog, specs, best["tile"], with_rgba8=True)
         grid = best["grid"]
 

@@ -122,13 +122,20 @@ def _signature(specs: Sequence[BatchJobSpec]) -> str:
 BLOCKS = (64, 128, 256, 512, 1024)
 TILE_MULTS = (1, 2, 4)
 CU_COUNT = 256
+# Program options tried per block size: the default (bounds staged in LDS
+# per tile) and, for the larger blocks, the per-job FP64 bounds read
+# through the scalar cache (same images).
+SCALAR_BOUNDS = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_SCALAR_BOUNDS=1"
+OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (256, 512, 1024)))
 
 
 def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mults=TILE_MULTS,
-             grid_waves_per_cu=(8, 16), reps: int = 2, cache_path: Optional[str] = None) -> Dict:
-    """Times one frame of `specs` for every (block, tile, grid) candidate and
-    returns the fastest with the whole table. grid = blocks such that each
-    CU gets the given number of waves."""
+             grid_waves_per_cu=(8, 16), reps: int = 2, cache_path: Optional[str] = None,
+             options=OPTIONS) -> Dict:
+    """Times one frame of `specs` for every (program options, block, tile,
+    grid) candidate and returns the fastest with the whole table. grid =
+    blocks such that each CU gets the given number of waves; `options` pairs
+    program options with the block sizes they are tried at."""
     key = f"gfx950:{_signature(specs)}"
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as f:
@@ -136,8 +143,9 @@ def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mu
         if key in known:
             return known[key]
     table = []
-    for block in blocks:
-        bprog = BatchProgram(ctx, block)
+    candidates = [(opts, block) for opts, opt_blocks in options for block in blocks if block in opt_blocks]
+    for opts, block in candidates:
+        bprog = BatchProgram(ctx, block, fp_options=opts)
         for m in tile_mults:
             tile = block * m
             batch = Batch(ctx, bprog, specs, tile)
@@ -151,7 +159,7 @@ def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mu
                     batch.render(grid)
                     ctx.sync()
                     times.append(time.perf_counter() - t0)
-                table.append({"block": block, "tile": tile, "grid": grid, "ms": 1e3 * min(times)})
+                table.append({"block": block, "tile": tile, "grid": grid, "options": opts, "ms": 1e3 * min(times)})
             batch.close()
     best = min(table, key=lambda r: r["ms"])
     result = {"best": best, "table": table, "paths": sum(s.width * s.height * s.spp for s in specs)}

@@ -89,14 +89,16 @@ def precompile_batch_programs(verbose: bool = False) -> None:
     programs (each batch scene type with HOPS = 4..12) that render jobs one
     at a time for comparison."""
     from . import scenes
-    from .batch import BLOCKS, precompile_batch
+    from .batch import BLOCKS, OPTIONS, precompile_batch
     from .objects import SceneImpl
     from .processing import precompile
     from .types import X32_F32
 
     t = time.time()
-    for block in BLOCKS:
-        precompile_batch(X32_F32, block)
+    for opts, blocks in OPTIONS:
+        for block in BLOCKS:
+            if block in blocks:
+                precompile_batch(X32_F32, block, fp_options=opts)
     rng = __import__("numpy").random.default_rng(0)
     protos = [scenes.batch_euclidean_scene(rng, 8), scenes.batch_hyperbolic_scene(rng, 9)]
     for s in protos:

@@ -21,8 +21,8 @@ namespace {
 
 // Mirror of ht::BatchJob in device/ht/batch.hh (64 bytes).
 struct DevJob {
-    uint64_t scene, canvas, seeds;
-    uint32_t type, w, h, spp, depth, tile_begin, tiles, pad0, pad1, pad2;
+    uint64_t scene, canvas, seeds, bounds;
+    uint32_t type, w, h, spp, depth, tile_begin, tiles, pad0;
 };
 static_assert(sizeof(DevJob) == 64, "BatchJob layout");
 
@@ -44,6 +44,7 @@ struct ht_batch {
     uint32_t tile = 0, block = 0, total_tiles = 0;
     DevJob *dev_jobs = nullptr;
     uint32_t *counter = nullptr;
+    void *bounds_mem = nullptr;  // scalar-bounds programs: every job's bounds array
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -78,6 +79,12 @@ int batch_frame(ht_batch *b, uint32_t grid, int clear, float gamma, int post) {
     HIP_TRY(hipGetLastError());
     uint32_t njobs = (uint32_t)b->jobs.size();
     void *jp = b->dev_jobs, *cp = b->counter;
+    if (b->prog->batch_bounds_fill) {
+        // the scenes' bounds, per frame (a caller may rewrite scene buffers between frames)
+        void *fill_args[] = {&jp, &njobs};
+        HIP_TRY(hipModuleLaunchKernel(b->prog->batch_bounds_fill, njobs, 1, 1, 256, 1, 1, 0, st, fill_args,
+                                      nullptr));
+    }
     void *args[] = {&jp, &njobs, &cp, &b->total_tiles, &b->tile};
     HIP_TRY(hipModuleLaunchKernel(b->prog->render_batch, grid, 1, 1, b->block, 1, 1, 0, st, args, nullptr));
     if (post) {
@@ -138,12 +145,24 @@ extern "C" int ht_batch_create(ht_prog *prog, uint32_t njobs, const ht_batch_job
         delete b;
         return fail(HT_ERR_BACKEND, std::string("ht_batch_create: hipMalloc: ") + hipGetErrorString(e));
     }
+    if (prog->batch_bounds_fill) {
+        size_t per_job = (size_t)prog->batch_bounds_cap * 64;
+        e = hipMalloc(&b->bounds_mem, per_job * njobs);
+        if (e != hipSuccess) {
+            (void)hipFree(mem);
+            delete b;
+            return fail(HT_ERR_BACKEND, std::string("ht_batch_create: hipMalloc (bounds): ") + hipGetErrorString(e));
+        }
+        for (uint32_t i = 0; i < njobs; ++i)
+            table[i].bounds = (uint64_t)(uintptr_t)((uint8_t *)b->bounds_mem + per_job * i);
+    }
     b->dev_jobs = (DevJob *)mem;
     b->counter = (uint32_t *)((uint8_t *)mem + sizeof(DevJob) * njobs);
     e = hipMemcpyAsync(b->dev_jobs, table.data(), sizeof(DevJob) * njobs, hipMemcpyHostToDevice, b->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->ctx->stream);
     if (e != hipSuccess) {
         (void)hipFree(mem);
+        if (b->bounds_mem) (void)hipFree(b->bounds_mem);
         delete b;
         return fail(HT_ERR_BACKEND, std::string("ht_batch_create: ") + hipGetErrorString(e));
     }
@@ -159,6 +178,7 @@ extern "C" void ht_batch_destroy(ht_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     (void)hipFree(b->dev_jobs);
+    if (b->bounds_mem) (void)hipFree(b->bounds_mem);
     (void)hipGetLastError();
     ht_ctx *ctx = b->ctx;
     delete b;

@@ -5,7 +5,9 @@
 // multiple of the block size),
 // numbered job after job (BatchJob::tile_begin). Each block takes the next
 // tile from a global counter, stages that job's scene bounds for the culled
-// scan (TopScan::prepare), and renders the tile's pixels, `spp` passes each
+// scan (TopScan::prepare; with HT_SCALAR_BOUNDS it publishes the address of
+// the job's FP64 bounds array, filled by batch_bounds_fill before the frame,
+// instead), and renders the tile's pixels, `spp` passes each
 // with in-thread path regeneration and at most `depth` hits per path; then it
 // takes the next tile. Blocks that finish early take more tiles, so uneven
 // jobs balance dynamically; the launch ends when the counter passes the last
@@ -13,9 +15,6 @@
 // HOPS = depth, so every job's canvas and seeds are bit-identical to
 // rendering it alone.
 #pragma once
-#ifdef HT_SCALAR_BOUNDS
-#error "batch programs scan a different scene per job; HT_SCALAR_BOUNDS keeps one bounds array per program"
-#endif
 #include "render.hh"
 
 #ifndef HT_BATCH_BLOCK
@@ -27,17 +26,22 @@ namespace ht {
 static_assert(HT_BATCH_BLOCK % 64 == 0 && HT_BATCH_BLOCK <= 64 * HT_MAX_WAVES,
               "the scan's per-wave LDS scratch must cover every wave of a batch block");
 
-// One job, written by the host (64 bytes; device pointers as integers).
+// One job, written by the host (64 bytes; device pointers as integers;
+// `bounds`: the job's BOUNDS_CAP FP64 bounds with HT_SCALAR_BOUNDS).
 struct BatchJob {
-    unsigned long long scene, canvas, seeds;
-    uint type, w, h, spp, depth, tile_begin, tiles, pad0, pad1, pad2;
+    unsigned long long scene, canvas, seeds, bounds;
+    uint type, w, h, spp, depth, tile_begin, tiles, pad0;
 };
 static_assert(sizeof(BatchJob) == 64, "BatchJob layout");
 
 template <class S>
 HD void batch_tile(const BatchJob &job, uint tile, uint tile_px) {
     const u8 *scene = reinterpret_cast<const u8 *>(job.scene);
+#ifdef HT_SCALAR_BOUNDS
+    if (threadIdx.x == 0) batch_bounds_slot() = job.bounds;
+#else
     TopScan<typename S::Object>::prepare(S::object(scene));
+#endif
     __syncthreads();
     uint npx = job.w * job.h;
     int hops = (int)job.depth < S::HOPS ? (int)job.depth : S::HOPS;
@@ -48,6 +52,19 @@ HD void batch_tile(const BatchJob &job, uint tile, uint tile_px) {
                                    reinterpret_cast<uint *>(job.seeds), job.spp, hops);
     }
     __syncthreads();  // the bounds are restaged for the next tile
+}
+
+// Job b's bounds (block b; any block size): HT_SCALAR_BOUNDS only.
+template <class S0, class S1> HD void batch_bounds_fill_impl(const BatchJob *__restrict__ jobs, uint njobs) {
+    if (blockIdx.x >= njobs) return;
+    const BatchJob job = jobs[blockIdx.x];
+    const u8 *scene = reinterpret_cast<const u8 *>(job.scene);
+    BoundS *out = reinterpret_cast<BoundS *>(job.bounds);
+    if (job.type == 0) {
+        TopScan<typename S0::Object>::fill_scalar(S0::object(scene), out, threadIdx.x, blockDim.x);
+    } else {
+        TopScan<typename S1::Object>::fill_scalar(S1::object(scene), out, threadIdx.x, blockDim.x);
+    }
 }
 
 template <class S0, class S1>
@@ -73,7 +90,20 @@ HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *_
 
 }  // namespace ht
 
+// HT_SCALAR_BOUNDS batch programs also export batch_bounds_fill(jobs,
+// njobs) (one block per job) and the capacity of a job's bounds array.
+#ifdef HT_SCALAR_BOUNDS
+#define HT_DEFINE_BATCH_BOUNDS_KERNEL(S0, S1)                                                            \
+    extern "C" __device__ unsigned ht_batch_bounds_cap = HT_BOUNDS_CAP;                                  \
+    extern "C" __global__ void __launch_bounds__(256)                                                    \
+        batch_bounds_fill(const ht::BatchJob *__restrict__ jobs, unsigned njobs) {                       \
+        ht::batch_bounds_fill_impl<S0, S1>(jobs, njobs);                                                 \
+    }
+#else
+#define HT_DEFINE_BATCH_BOUNDS_KERNEL(S0, S1)
+#endif
 #define HT_DEFINE_BATCH_KERNEL(S0, S1)                                                                   \
+    HT_DEFINE_BATCH_BOUNDS_KERNEL(S0, S1)                                                                \
     extern "C" __global__ void __launch_bounds__(HT_BATCH_BLOCK)                                         \
         render_batch(const ht::BatchJob *__restrict__ jobs, unsigned njobs, unsigned *__restrict__ counter, \
                      unsigned total_tiles, unsigned tile_px) {                                           \

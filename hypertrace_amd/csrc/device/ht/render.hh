@@ -182,7 +182,9 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 #ifdef HT_SCALAR_BOUNDS
 #define HT_DEFINE_BOUNDS_KERNEL(S)                                                                          \
     extern "C" __global__ void __launch_bounds__(256) bounds_fill(const unsigned char *__restrict__ scene) { \
-        ht::TopScan<typename S::Object>::fill_scalar(S::object(scene));                                     \
+        ht::TopScan<typename S::Object>::fill_scalar(S::object(scene), ht::ht_scalar_bounds,                \
+                                                     blockIdx.x * blockDim.x + threadIdx.x,                 \
+                                                     gridDim.x * blockDim.x);                               \
     }
 #else
 #define HT_DEFINE_BOUNDS_KERNEL(S)

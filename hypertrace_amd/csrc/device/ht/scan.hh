@@ -122,14 +122,32 @@ HD BoundS widen(const Bound &b) {
     return w;
 }
 #ifdef HT_SCALAR_BOUNDS
-// The top-level vector's bounds for the scalar-load scan; written by the
-// program's bounds_fill kernel before each render launch (launches of one
-// program on one stream are ordered, so one array per loaded program).
-__device__ BoundS ht_scalar_bounds[HT_BOUNDS_CAP];
-// The scan reads the array through a constant-address-space pointer (the
-// same memory; no kernel that reads it writes it), which lets wave-uniform
-// loads go through the scalar cache into SGPRs.
+// The scan reads the bounds through a constant-address-space pointer (the
+// same memory; no kernel that reads them writes them), which lets
+// wave-uniform loads go through the scalar cache into SGPRs.
 typedef __attribute__((address_space(4))) const BoundS ConstBoundS;
+#ifdef HT_BATCH_BLOCK
+// Batch programs: one array per job (filled by batch_bounds_fill before the
+// frame); the block publishes the address of its current job's array in
+// LDS before the tile's paths run, and the scan reads it as a wave-uniform
+// (SGPR) base.
+HD unsigned long long &batch_bounds_slot() {
+    __shared__ unsigned long long slot;
+    return slot;
+}
+HD const ConstBoundS *scalar_bounds_base() {
+    unsigned long long v = batch_bounds_slot();
+    uint lo = __builtin_amdgcn_readfirstlane((uint)v);
+    uint hi = __builtin_amdgcn_readfirstlane((uint)(v >> 32));
+    return (const ConstBoundS *)(((unsigned long long)hi << 32) | lo);
+}
+#else
+// Scene programs: the top-level vector's bounds, written by the program's
+// bounds_fill kernel before each render launch (launches of one program on
+// one stream are ordered, so one array per loaded program).
+__device__ BoundS ht_scalar_bounds[HT_BOUNDS_CAP];
+HD const ConstBoundS *scalar_bounds_base() { return (const ConstBoundS *)ht_scalar_bounds; }
+#endif
 // One 64-byte scalar load per bound (s_load_dwordx16), whatever fields the
 // test uses.
 typedef unsigned int u32x16 __attribute__((ext_vector_type(16)));
@@ -553,7 +571,7 @@ HD unsigned long long hit_key(float d, usize i) {
 // elements in FP32 programs, the element-by-element scan otherwise.
 template <class T> struct TopScan {
     static HD void prepare(const u8 *) {}
-    static HD void fill_scalar(const u8 *) {}
+    static HD void fill_scalar(const u8 *, BoundS *, uint, uint) {}
     template <class G>
     static HD real detect(const u8 *self, Context *ctx, typename T::Cache *cache, Light<G> *light) {
         return T::detect(self, ctx, cache, light);
@@ -594,19 +612,18 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
 #endif
     }
 
-    // The bounds of the scalar-load scan, from any grid (grid-stride loop);
-    // a scene with more elements than the array holds is scanned unculled.
-    static HD void fill_scalar(const u8 *self) {
-#ifdef HT_SCALAR_BOUNDS
+    // The bounds of the scalar-load scan into `out`, element i by thread
+    // first + k * stride; a scene with more elements than BOUNDS_CAP is
+    // scanned unculled.
+    static HD void fill_scalar(const u8 *self, BoundS *out, uint first, uint stride) {
         if constexpr (CULL) {
             usize n = V::len(self);
             if (n > (usize)BOUNDS_CAP) return;
-            for (uint i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint)n; i += gridDim.x * blockDim.x)
-                ht_scalar_bounds[i] = widen(element_bound<E, typename E::Map>(V::elem(self, i)));
+            for (uint i = first; i < (uint)n; i += stride)
+                out[i] = widen(element_bound<E, typename E::Map>(V::elem(self, i)));
+        } else {
+            (void)self, (void)out, (void)first, (void)stride;
         }
-#else
-        (void)self;
-#endif
     }
 
     // All threads of the block (see TreeLds).
@@ -818,7 +835,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             usize n = V::len(self);
             if (n > (usize)BOUNDS_CAP) return OV::detect(self, ctx, cache, light);
 #ifdef HT_SCALAR_BOUNDS
-            const ConstBoundS *sb = (const ConstBoundS *)ht_scalar_bounds;
+            const ConstBoundS *sb = scalar_bounds_base();
             auto bound = [&](usize e) { return load_bound(sb + e); };
 #else
             const Bound *lb = bounds_lds<OV>();

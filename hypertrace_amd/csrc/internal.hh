@@ -43,6 +43,10 @@ struct ht_prog {
     hipFunction_t render_passes = nullptr;
     hipFunction_t render_cached = nullptr;  // null unless the scene type supports the radiance cache
     hipFunction_t render_batch = nullptr;   // batch programs (several scene types, one persistent kernel)
+    // scalar-bounds batch programs: fills every job's bounds array before a
+    // frame; each array holds batch_bounds_cap bounds of 64 B
+    hipFunction_t batch_bounds_fill = nullptr;
+    unsigned batch_bounds_cap = 0;
     hipFunction_t bounds_fill = nullptr;    // scalar-bounds programs: fills the bounds before each render launch
     // Wavefront form (programs whose root source adds the wavefront kernels):
     // kernels, queue sizes {extend-record words, shade-record words, keys,

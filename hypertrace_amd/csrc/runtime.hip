@@ -451,6 +451,18 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
     p->render_cached = find("render_cached");
     p->render_batch = find("render_batch");
     p->bounds_fill = find("bounds_fill");
+    p->batch_bounds_fill = find("batch_bounds_fill");
+    if (p->batch_bounds_fill) {
+        hipDeviceptr_t d = nullptr;
+        size_t bytes = 0;
+        e = hipModuleGetGlobal(&d, &bytes, p->module, "ht_batch_bounds_cap");
+        if (e == hipSuccess && bytes == sizeof(unsigned)) e = hipMemcpyDtoH(&p->batch_bounds_cap, d, bytes);
+        if (e != hipSuccess || bytes != sizeof(unsigned) || p->batch_bounds_cap == 0) {
+            (void)hipModuleUnload(p->module);
+            delete p;
+            return fail(HT_ERR_BACKEND, "batch program has batch_bounds_fill but no readable ht_batch_bounds_cap");
+        }
+    }
     p->wf_generate = find("wf_generate");
     p->wf_extend = find("wf_extend");
     p->wf_shade = find("wf_shade");

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from hypertrace_amd import scenes
-from hypertrace_amd.batch import Batch, BatchProgram
+from hypertrace_amd.batch import SCALAR_BOUNDS, Batch, BatchProgram
 from hypertrace_amd.native import HtError
 from hypertrace_amd.objects import SceneImpl
 from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
@@ -36,11 +36,15 @@ def alone(ctx, spec, frames=1):
     return c.load_image(), c.load_seeds()
 
 
-@pytest.mark.parametrize("block,mult", [(256, 1), (64, 4), (1024, 1)])
-def test_batch_jobs_identical_to_single_renders(ctx, block, mult):
+@pytest.mark.parametrize("block,mult,opts", [(256, 1, None), (64, 4, None), (1024, 1, None),
+                                             (256, 1, SCALAR_BOUNDS), (1024, 2, SCALAR_BOUNDS)])
+def test_batch_jobs_identical_to_single_renders(ctx, block, mult, opts):
+    """Every job's canvas and seeds equal rendering it alone, with the bounds
+    staged in LDS per tile or (scalar bounds) read from the job's own FP64
+    array."""
     specs = scenes.batch_jobs(**SMALL)
-    print(f"case block {block} tile {block * mult}", flush=True)
-    bprog = BatchProgram(ctx, block)
+    print(f"case block {block} tile {block * mult} opts {opts}", flush=True)
+    bprog = BatchProgram(ctx, block, fp_options=opts)
     batch = Batch(ctx, bprog, specs, block * mult)
     batch.render(grid=7, clear=False)
     ctx.sync()
@@ -51,9 +55,10 @@ def test_batch_jobs_identical_to_single_renders(ctx, block, mult):
         np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
 
 
-def test_graph_replays_match_eager_frames(ctx):
+@pytest.mark.parametrize("opts", [None, SCALAR_BOUNDS])
+def test_graph_replays_match_eager_frames(ctx, opts):
     specs = scenes.batch_jobs(**SMALL)
-    bprog = BatchProgram(ctx, 256)
+    bprog = BatchProgram(ctx, 256, fp_options=opts)
     eager = Batch(ctx, bprog, specs, with_rgba8=True)
     graph = Batch(ctx, bprog, specs, with_rgba8=True)
     graph.capture(grid=16, clear=True, post=True)
