@@ -123,10 +123,11 @@ BLOCKS = (64, 128, 256, 512, 1024)
 TILE_MULTS = (1, 2, 4)
 CU_COUNT = 256
 # Program options tried per block size: the default (bounds staged in LDS
-# per tile) and, for the larger blocks, the per-job FP64 bounds read
-# through the scalar cache (same images).
+# per tile) and, from 128 threads up, the per-job FP64 bounds read through
+# the scalar cache (same images; without bounds in LDS the register cap of
+# 4 waves per SIMD sets the occupancy).
 SCALAR_BOUNDS = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_SCALAR_BOUNDS=1"
-OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (256, 512, 1024)))
+OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (128, 256, 512, 1024)))
 
 
 def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mults=TILE_MULTS,

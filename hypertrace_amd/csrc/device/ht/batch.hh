@@ -104,7 +104,7 @@ HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *_
 #endif
 #define HT_DEFINE_BATCH_KERNEL(S0, S1)                                                                   \
     HT_DEFINE_BATCH_BOUNDS_KERNEL(S0, S1)                                                                \
-    extern "C" __global__ void __launch_bounds__(HT_BATCH_BLOCK)                                         \
+    extern "C" __global__ void __launch_bounds__(HT_BATCH_BLOCK, HT_WAVES_PER_EU)                        \
         render_batch(const ht::BatchJob *__restrict__ jobs, unsigned njobs, unsigned *__restrict__ counter, \
                      unsigned total_tiles, unsigned tile_px) {                                           \
         ht::render_batch_impl<S0, S1>(jobs, njobs, counter, total_tiles, tile_px);                       \
