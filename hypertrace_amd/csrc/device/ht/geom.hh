@@ -201,10 +201,21 @@ struct Hy {
     static HD Pos origin() { return r4(R0, R0, R1, R0); }
     static HD real3 dir_to_local(Pos, Dir d) { return xyz(d); }
     static HD Dir dir_from_local(Pos, real3 d) { return q_new3(d, R0); }
-    // hyperbolic.cc:20-23
+    // hyperbolic.cc:20-23. Program option HT_HY_DIST_FP64 (a precision
+    // method, not the drop-in: results differ from the reference in the last
+    // bits): the same distance in FP64 through the cancellation-free form
+    // acosh(1 + u) = log1p(u + sqrt(u (u + 2))), u = |a - b|^2 / (2 a.z b.z),
+    // which keeps its relative accuracy both for nearby points (u -> 0,
+    // where x * x - 1 cancels) and near the boundary (a.z b.z -> 0).
     static HD real distance(quat a, quat b) {
+#ifdef HT_HY_DIST_FP64
+        double dx = (double)a.x - (double)b.x, dy = (double)a.y - (double)b.y, dz = (double)a.z - (double)b.z;
+        double u = (dx * dx + dy * dy + dz * dz) / (2.0 * (double)a.z * (double)b.z);
+        return (real)::log1p(u + ::sqrt(u * (u + 2.0)));
+#else
         real x = 1 + length2(a - b) / (2 * a.z * b.z);
         return log_(x + sqrt_(x * x - 1));
+#endif
     }
     // hyperbolic.cc:25-33
     static HD Dir dir_at(quat p, quat d, quat h) {

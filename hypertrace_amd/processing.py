@@ -212,13 +212,14 @@ class Program:
 
 
 def program_options(fast: bool = False, rr_depth: Optional[int] = None, no_cull: bool = False,
-                    scalar_bounds: bool = False) -> str:
+                    scalar_bounds: bool = False, hy_distance_fp64: bool = False) -> str:
     """Program build options: strict (default) or fast FP32; Russian roulette
     from the rr_depth-th hit (a different method: it draws one more uniform
     per hit from there on); the element-by-element scan (comparisons); the
     culled scan's bounds in a global FP64 array read through the scalar
     cache instead of LDS (same images; frees LDS on scenes with many
-    elements)."""
+    elements); the hyperbolic distance in FP64 with a cancellation-free
+    acosh (a precision method: not bit-identical to the reference)."""
     opts = FAST_FP if fast else STRICT_FP
     if rr_depth is not None:
         opts += f" -DHT_RR_DEPTH={int(rr_depth)}"
@@ -226,6 +227,8 @@ def program_options(fast: bool = False, rr_depth: Optional[int] = None, no_cull:
         opts += " -DHT_NO_CULL=1"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
+    if hy_distance_fp64:
+        opts += " -DHT_HY_DIST_FP64=1"
     return opts
 
 

@@ -39,8 +39,8 @@ def within(got, ref):
     return np.all(np.abs(got - ref) <= REL * np.abs(ref) + ABS, axis=-1)
 
 
-def one_pass(ctx, wl, passes=1):
-    render = Render(ctx, wl.scene)
+def one_pass(ctx, wl, passes=1, fp_options=None):
+    render = Render(ctx, wl.scene, fp_options=fp_options)
     buf = EntityBuffer(ctx, wl.scene)
     canvas = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
     seeds0 = canvas.load_seeds().copy()
@@ -83,6 +83,29 @@ def test_single_pass_parity_f64(ctx64, name):
                                       precision=oracle.F64)
     ok = within(got, ref)
     assert ok.mean() >= 0.999, f"{name}: only {ok.mean():.5f} of pixels within 1e-4"
+
+
+@pytest.mark.parametrize("name", ["hy_cells", "hy_example"])
+def test_hy_distance_fp64_option(ctx, name):
+    """Program option HT_HY_DIST_FP64 (the hyperbolic distance in FP64 with a
+    cancellation-free acosh; a precision method, not the drop-in). Against
+    the FP64 oracle with the same seeds its single-sample radiance agrees at
+    least as often as the reference's FP32 formula does, and it stays with
+    the FP32 oracle on nearly every pixel (the distance only decides near
+    ties and short hops)."""
+    from hypertrace_amd.processing import program_options
+
+    wl = WORKLOADS[name]()
+    print(f"case {name} hy_distance_fp64", flush=True)
+    seeds0, base, _ = one_pass(ctx, wl)
+    _, opt, _ = one_pass(ctx, wl, fp_options=program_options(hy_distance_fp64=True))
+    ref32, _, _ = oracle.render(wl.scene.spec_text(X32_F32), wl.width, wl.height, 1, seeds0)
+    ref64, _, _ = oracle.render(wl.scene.spec_text(X32_F64), wl.width, wl.height, 1, seeds0, precision=oracle.F64)
+    base64, opt64, opt32 = within(base, ref64).mean(), within(opt, ref64).mean(), within(opt, ref32).mean()
+    print(f"within 1e-4 of FP64: reference formula {base64:.5f}, FP64 distance {opt64:.5f}; "
+          f"FP64 distance within 1e-4 of the FP32 oracle {opt32:.5f}")
+    assert opt64 >= base64 - 0.002
+    assert opt32 >= 0.98
 
 
 @pytest.mark.parametrize("name", ["eu_ball", "hy_cells"])
