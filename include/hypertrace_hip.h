@@ -203,7 +203,11 @@ int ht_cache_train(ht_cache *cache, uint32_t steps, uint32_t batch, float lr);
  * a multiple of the block size) is chosen at ht_batch_create. Per pixel the
  * work equals ht_render_passes of the job's scene with at most `depth` hits
  * per path, so canvases and seeds are bit-identical to rendering each job
- * alone. */
+ * alone. A batch program built with -DHT_SCALAR_BOUNDS=1 reads each job's
+ * culling bounds from its own FP64 array (allocated by ht_batch_create,
+ * 64 B x the program's bounds capacity per job, filled at the start of every
+ * frame by the program's batch_bounds_fill kernel) instead of staging them
+ * in LDS per tile; same images. */
 typedef struct ht_batch ht_batch;
 typedef struct ht_batch_job {
     uint32_t type;             /* 0 or 1: Scene0 or Scene1 of the batch program */
