@@ -207,14 +207,19 @@ def cache_leg(args):
         buf = EntityBuffer(ctx, wl.scene)
         compose_on_device(ctx, buf, wl.scene, wl.word_maps)
         from hypertrace_amd.tuning import tune_render
-        tuning = tune_render(ctx, wl.scene, W, H, 8, wl.seed_mode, wl.seed)
+        # 32-spp frames: the timed launch runs 150+ passes, over which each
+        # lane's work evens out, so very short frames overrate the pixel queue
+        tuning = tune_render(ctx, wl.scene, W, H, 32, wl.seed_mode, wl.seed)
         chosen = Render(ctx, wl.scene, fp_options=tuning["best"]["options"])
         plain = None if args.no_plain else Render(ctx, wl.scene, fp_options=NO_CULL)
         base = time_to_rmse(ctx, wl, buf, chosen, plain, args, reference)
         cache = RadianceCache(ctx, W * H, 65536, 16384)
-        # the cached kernel runs the same scan: build it with the tuner's options
+        # the cached kernel runs the same scan: build it with the tuner's best
+        # options among those that apply to it (the pixel queue only changes
+        # render_passes; the cached kernel runs one pass per launch)
+        cached_knobs = min((t for t in tuning["table"] if not t["pixel_queue"]), key=lambda t: t["ms"])
         cr = CachedRender(ctx, wl.scene, cache, query_hop=2, train_every=16, steps_per_pass=4, batch=16384,
-                          lr=1e-3, fp_options=tuning["best"]["options"])
+                          lr=1e-3, fp_options=cached_knobs["options"])
         canvas = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
         t, spp, curve, reached, check = 0.0, 0, [], None, 4
         while spp < args.cache_max_spp:
@@ -234,7 +239,7 @@ def cache_leg(args):
     out = {"workload": "hyperbolic_cells 1920x1080, 256 primitives, 8 hops", "target": args.target_rmse,
            "oracle_spp": reference[3], "oracle_noise_rmse": reference[1], "oracle_render_s": reference[2],
            "bit_exact_forms": {k: base.get(k) for k in ("reached", "spp", "chosen_s", "plain_s", "speedup")},
-           "chosen_program": tuning["best"],
+           "chosen_program": tuning["best"], "cached_program": cached_knobs, "render_tuning": tuning,
            "cached": {"reached": reached is not None, "spp": reached if reached is not None else spp,
                       "seconds": t, "ms_per_pass": t * 1e3 / max(spp, 1), "curve": curve[::4] + curve[-1:],
                       "queries_last_pass": queries, "training_records_last_pass": records,
@@ -248,15 +253,18 @@ def cache_leg(args):
 
 
 def fixed_knobs(text, base=None):
-    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN]" (no timing)."""
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q]" (no timing)."""
     from hypertrace_amd.processing import STRICT_FP
     from hypertrace_amd.tuning import options
     knobs = text.split(",")
     unroll, waves = int(knobs[0]), int(knobs[1])
     sb = "sb" in knobs[2:]
     groups = next((int(k[1:]) for k in knobs[2:] if k.startswith("g")), 0)
+    queue = "q" in knobs[2:]
     return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                     "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups)},
+                     "pixel_queue": queue,
+                     "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups,
+                                        pixel_queue=queue)},
             "table": [], "fixed": True}
 
 
@@ -428,10 +436,10 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
-    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN]",
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q]",
                     help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
                          "instead of tuning (keeps a profiled trace to one program)")
-    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN]",
+    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q]",
                     help="the same for the SDF leg's program")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "

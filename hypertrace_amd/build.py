@@ -62,7 +62,7 @@ def precompile_programs(verbose: bool = False) -> None:
     from .processing import FAST_FP, NO_CULL_FP, precompile, program_options
     from .types import X32_F32, X32_F64
 
-    from .tuning import CANDIDATES, options
+    from .tuning import CANDIDATES, candidate_options
 
     rr = program_options(rr_depth=3)
     tree = program_options() + " -DHT_CULL_TREE=1"
@@ -72,10 +72,10 @@ def precompile_programs(verbose: bool = False) -> None:
         if k in (2, 5):  # the cell scene and the 1024-shape scene: the sphere-tree option
             extra.append((X32_F32, tree))
         if k == 2:  # the cell scene: the render tuner's candidates
-            extra += [(X32_F32, options(*c[:2], scalar_bounds=c[2], groups=c[3])) for c in CANDIDATES]
+            extra += [(X32_F32, candidate_options(c)) for c in CANDIDATES]
             extra.append((X32_F32, program_options(scalar_bounds=True)))
         if k == 5:  # the 1024-shape scene: the tuner's candidates with Russian roulette
-            extra += [(X32_F32, options(*c[:2], base=rr, scalar_bounds=c[2], groups=c[3])) for c in CANDIDATES]
+            extra += [(X32_F32, candidate_options(c, rr)) for c in CANDIDATES]
         for cfg, fp in [(X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP), (X32_F32, NO_CULL_FP)] + extra:
             t = time.time()
             hit = precompile(scene, cfg, fp_options=fp)

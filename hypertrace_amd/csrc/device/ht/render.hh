@@ -156,14 +156,108 @@ HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ sce
     seeds[idx] = ctx.rng.state;
 }
 
+#ifdef HT_PIXEL_QUEUE
+// Program option HT_PIXEL_QUEUE: render_passes as a persistent kernel (the
+// runtime launches only the blocks that fit on the device at once, after
+// pixel_queue_reset). A lane takes the next pixel from a global counter
+// (one atomic per wave for all lanes that need one) and runs its `passes`
+// samples with in-thread regeneration, exactly as render_pixel_passes does;
+// when they are done it writes the pixel back and takes another. Lanes whose
+// pixels need fewer hits no longer wait for the slowest lane of a static
+// wave, which matters when the frame has many pixels per lane. Same canvas
+// and seeds.
+__device__ uint ht_pixel_next;
+
+template <class S>
+HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
+                             uint *__restrict__ seeds, uint passes) {
+    typedef typename S::Geo G;
+    const uint n = w * h;
+    const int hops = S::HOPS;
+    const unsigned long long below = lane_mask_below();
+    bool have = false, drained = false, start = true;
+    uint idx = 0, done = 0;
+    int hop = 0;
+    PixelSetup px;
+    Context ctx;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    Light<G> light;
+    color3 emission = c3(0.0f);
+    for (;;) {
+        if (!drained) {
+            bool need = !have;
+            unsigned long long m = __ballot(need);
+            if (m != 0) {
+                uint leader = (uint)__ffsll((long long)m) - 1u;
+                uint base = 0;
+                if (__lane_id() == leader) base = atomicAdd(&ht_pixel_next, (uint)__popcll(m));
+                base = __shfl(base, (int)leader);
+                if (base + (uint)__popcll(m) >= n) drained = true;  // the counter only grows
+                uint cand = base + (uint)__popcll(m & below);
+                if (need && cand < n) {
+                    have = true;
+                    idx = cand;
+                    px = pixel_setup(idx % w, idx / w, w, h);
+                    ctx.rng.state = seeds[idx];
+                    ctx.hasher = hash_init();
+                    acc = canvas[idx];
+                    done = 0;
+                    start = true;
+                    if (hops <= 0) {
+                        for (uint k = 0; k < passes; ++k) {
+                            start_path<S>(scene, &ctx, px, &light, &emission);
+                            acc.w += 1.0f;
+                        }
+                        canvas[idx] = acc;
+                        seeds[idx] = ctx.rng.state;
+                        have = false;
+                    }
+                }
+            }
+        }
+        if (__ballot(have) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (have) {
+            if (start) {
+                start_path<S>(scene, &ctx, px, &light, &emission);
+                hop = 0;
+                start = false;
+            }
+            bool end = bounce<S>(scene, &ctx, &light, &emission, hop);
+            ++hop;
+            if (hop >= hops) end = true;
+            if (end) {
+                acc.x += emission.x;
+                acc.y += emission.y;
+                acc.z += emission.z;
+                acc.w += 1.0f;
+                ++done;
+                start = true;
+                if (done == passes) {
+                    canvas[idx] = acc;
+                    seeds[idx] = ctx.rng.state;
+                    have = false;
+                }
+            }
+        }
+    }
+}
+#endif
+
 template <class S>
 HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
                            uint *__restrict__ seeds, uint passes) {
     TopScan<typename S::Object>::prepare(S::object(scene));
     __syncthreads();
+#ifdef HT_PIXEL_QUEUE
+    render_passes_queued<S>(w, h, scene, canvas, seeds, passes);
+#else
     uint idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= w * h) return;
     render_pixel_passes<S>(w, h, idx, scene, canvas, seeds, passes, S::HOPS);
+#endif
 }
 
 }  // namespace ht
@@ -189,8 +283,17 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 #else
 #define HT_DEFINE_BOUNDS_KERNEL(S)
 #endif
+#ifdef HT_PIXEL_QUEUE
+#define HT_DEFINE_QUEUE_KERNEL                                                                        \
+    extern "C" __global__ void pixel_queue_reset() {                                                  \
+        if (threadIdx.x == 0) ht::ht_pixel_next = 0u;                                                 \
+    }
+#else
+#define HT_DEFINE_QUEUE_KERNEL
+#endif
 #define HT_DEFINE_RENDER_KERNELS(S)                                                                   \
     HT_DEFINE_BOUNDS_KERNEL(S)                                                                        \
+    HT_DEFINE_QUEUE_KERNEL                                                                            \
     extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU) render(                        \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas, \
         unsigned *__restrict__ seeds) {                                                               \

@@ -48,6 +48,10 @@ struct ht_prog {
     hipFunction_t batch_bounds_fill = nullptr;
     unsigned batch_bounds_cap = 0;
     hipFunction_t bounds_fill = nullptr;    // scalar-bounds programs: fills the bounds before each render launch
+    // pixel-queue programs: resets the pixel counter before each
+    // render_passes launch, which then runs as many blocks as fit at once
+    hipFunction_t pixel_queue_reset = nullptr;
+    uint32_t resident_blocks = 0;
     // Wavefront form (programs whose root source adds the wavefront kernels):
     // kernels, queue sizes {extend-record words, shade-record words, keys,
     // counter words, HOPS} and the queue workspace, grown on demand.

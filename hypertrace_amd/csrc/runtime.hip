@@ -451,6 +451,18 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
     p->render_cached = find("render_cached");
     p->render_batch = find("render_batch");
     p->bounds_fill = find("bounds_fill");
+    p->pixel_queue_reset = find("pixel_queue_reset");
+    if (p->pixel_queue_reset && p->render_passes) {
+        int per_cu = 0, cus = 0;
+        e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->render_passes, 256, 0);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+        if (e != hipSuccess || per_cu <= 0 || cus <= 0) {
+            (void)hipModuleUnload(p->module);
+            delete p;
+            return fail(HT_ERR_BACKEND, "pixel-queue program: occupancy query failed");
+        }
+        p->resident_blocks = (uint32_t)(per_cu * cus);
+    }
     p->batch_bounds_fill = find("batch_bounds_fill");
     if (p->batch_bounds_fill) {
         hipDeviceptr_t d = nullptr;
@@ -606,6 +618,11 @@ int launch_render(ht_prog *prog, hipFunction_t fn, uint32_t w, uint32_t h, ht_bu
     void *sp = scene->ptr, *cp = canvas->ptr, *pp = seeds->ptr;
     std::lock_guard<std::mutex> lock(prog->launch_mu);
     if ((rc = ht_internal::fill_bounds(prog, sp))) return rc;
+    if (with_passes && prog->pixel_queue_reset) {
+        // persistent: the blocks that fit at once pull pixels from the counter
+        HIP_TRY(hipModuleLaunchKernel(prog->pixel_queue_reset, 1, 1, 1, 64, 1, 1, 0, ctx->stream, nullptr, nullptr));
+        if (grid > prog->resident_blocks) grid = prog->resident_blocks;
+    }
     void *args_pass[] = {&w, &h, &sp, &cp, &pp};
     void *args_passes[] = {&w, &h, &sp, &cp, &pp, &passes};
     HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, ctx->stream,

@@ -8,10 +8,16 @@ against resident waves; HT_SCALAR_BOUNDS moves the element bounds from LDS
 (FP32, widened per lane) to a global FP64 array read through the scalar
 cache, which frees the LDS that limits occupancy on scenes with many
 elements; HT_CULL_GROUPS scans groups of Morton-ordered elements first and
-skips a group's elements when no lane of the wave passes its sphere. Every setting renders bit-identical images (only the schedule
-differs), so the tuner simply times one frame of the caller's workload per
-candidate on the GPU and keeps the fastest. Small frames (few waves per
-SIMD) prefer fewer registers less than large ones.
+skips a group's elements when no lane of the wave passes its sphere;
+HT_PIXEL_QUEUE launches only as many blocks as the device holds at once and
+has each lane take its next pixel from a global counter when its pixel's
+samples are done, which removes the tail of waves whose slowest pixel holds
+the other 63 lanes (frames with more pixels than resident lanes gain; a
+frame with fewer pixels than two rounds of resident lanes loses). Every
+setting renders bit-identical images (only the schedule differs), so the
+tuner simply times one frame of the caller's workload per candidate on the
+GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
+registers less than large ones.
 """
 from __future__ import annotations
 
@@ -21,33 +27,44 @@ from typing import Dict, Optional, Sequence, Tuple
 
 from .processing import STRICT_FP, Canvas, Context, EntityBuffer, Render
 
-# (cull unroll, waves per SIMD, scalar bounds, group size or 0)
-CANDIDATES: Tuple[Tuple[int, int, bool, int], ...] = (
-    (4, 4, False, 0), (4, 5, False, 0), (2, 5, False, 0), (4, 6, False, 0),
-    (4, 4, True, 0), (4, 5, True, 0), (4, 6, True, 0), (4, 5, False, 4))
+# (cull unroll, waves per SIMD, scalar bounds, group size or 0, pixel queue)
+CANDIDATES: Tuple[Tuple[int, int, bool, int, bool], ...] = (
+    (4, 4, False, 0, False), (4, 5, False, 0, False), (2, 5, False, 0, False), (4, 6, False, 0, False),
+    (4, 4, True, 0, False), (4, 5, True, 0, False), (4, 6, True, 0, False), (4, 5, False, 4, False),
+    (4, 4, False, 0, True), (4, 4, True, 0, True), (4, 6, True, 0, True))
 
 
-def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0) -> str:
+def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
+            pixel_queue: bool = False) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
     if groups:
         opts += f" -DHT_CULL_GROUPS={int(groups)}"
+    if pixel_queue:
+        opts += " -DHT_PIXEL_QUEUE=1"
     return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool]:
     unroll, waves = int(candidate[0]), int(candidate[1])
     sb = bool(candidate[2]) if len(candidate) > 2 else False
     groups = int(candidate[3]) if len(candidate) > 3 else 0
-    return unroll, waves, sb, groups
+    queue = bool(candidate[4]) if len(candidate) > 4 else False
+    return unroll, waves, sb, groups, queue
+
+
+def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
+    """Program options of one tuner candidate tuple."""
+    unroll, waves, sb, groups, queue = _knobs(candidate)
+    return options(unroll, waves, base, sb, groups, queue)
 
 
 def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mode: int = 1, seed: int = 0,
                 candidates: Sequence[Sequence] = CANDIDATES, reps: int = 3, base: str = STRICT_FP,
                 cache_path: Optional[str] = None) -> Dict:
     """Times `spp` passes of render_passes (HIP events, median of `reps`) for
-    each (unroll, waves[, scalar bounds[, groups]]) candidate; returns {"best": {...},
+    each (unroll, waves[, scalar bounds[, groups[, pixel queue]]]) candidate; returns {"best": {...},
     "table": [...]} with the program options of the best."""
     key = f"gfx950:{scene.cpp()}:{width}x{height}:{spp}:{base}"
     if cache_path and os.path.exists(cache_path):
@@ -58,8 +75,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb, groups = _knobs(cand)
-        opts = options(unroll, waves, base, sb, groups)
+        unroll, waves, sb, groups, queue = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups, queue)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -70,7 +87,7 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             r.render_passes(buf, c, spp)
             times.append(ctx.timer_end())
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                      "ms": sorted(times)[len(times) // 2], "options": opts})
+                      "pixel_queue": queue, "ms": sorted(times)[len(times) // 2], "options": opts})
     best = min(table, key=lambda t: t["ms"])
     result = {"best": best, "table": table}
     if cache_path:
