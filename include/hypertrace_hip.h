@@ -61,7 +61,12 @@ int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int 
  * default 4), -DHT_SCALAR_BOUNDS=1 (the scan's bounds in a global FP64
  * array read through the scalar cache, filled by the program's bounds_fill
  * kernel, which ht_render / ht_render_passes / ht_render_cached launch
- * first on the context's stream), -DHT_CULL_TREE=1 (sphere-tree walk); and
+ * first on the context's stream), -DHT_CULL_TREE=1 (sphere-tree walk),
+ * -DHT_CULL_GROUPS=g (two-level scan over groups of g elements),
+ * -DHT_PIXEL_QUEUE=1 (ht_render_passes launches only the blocks that fit
+ * on the device at once, after the program's pixel_queue_reset kernel, and
+ * lanes take pixels from a per-program counter; launches of one program
+ * are ordered on its context's stream, so they never share it); and
  * methods that do not: -DHT_RR_DEPTH=d (Russian roulette), -DHT_NO_CULL=1
  * (the element-by-element scan, same images, for comparisons). No
  * reference counterpart beyond ocl::Program build options. */
