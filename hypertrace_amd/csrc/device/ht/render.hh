@@ -178,7 +178,6 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
     bool have = false, drained = false, start = true;
     uint idx = 0, done = 0;
     int hop = 0;
-    PixelSetup px;
     Context ctx;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     Light<G> light;
@@ -197,13 +196,13 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
                 if (need && cand < n) {
                     have = true;
                     idx = cand;
-                    px = pixel_setup(idx % w, idx / w, w, h);
                     ctx.rng.state = seeds[idx];
                     ctx.hasher = hash_init();
                     acc = canvas[idx];
                     done = 0;
                     start = true;
                     if (hops <= 0) {
+                        PixelSetup px = pixel_setup(idx % w, idx / w, w, h);
                         for (uint k = 0; k < passes; ++k) {
                             start_path<S>(scene, &ctx, px, &light, &emission);
                             acc.w += 1.0f;
@@ -221,7 +220,8 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
         }
         if (have) {
             if (start) {
-                start_path<S>(scene, &ctx, px, &light, &emission);
+                // the pixel position is recomputed per sample: fewer live registers
+                start_path<S>(scene, &ctx, pixel_setup(idx % w, idx / w, w, h), &light, &emission);
                 hop = 0;
                 start = false;
             }

@@ -246,3 +246,24 @@ def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
         out.append((c.load_image(), c.load_seeds()))
     np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
+def test_pixel_queue_ragged_frames(ctx, size):
+    """The pixel-queue program on frames smaller than a block or with a
+    partial last wave, launched twice (the counter is reset before every
+    launch): the same canvas and seeds as the one-thread-per-pixel kernel."""
+    w, h = size
+    wl = scenes.euclidean_spheres(w, h)
+    print(f"case pixel queue {w}x{h}", flush=True)
+    buf = EntityBuffer(ctx, wl.scene)
+    out = []
+    for fp in (CULLED_VARIANTS["pixel_queue"], None):
+        r = Render(ctx, wl.scene, fp_options=fp)
+        c = Canvas(ctx, (w, h), wl.seed_mode, wl.seed)
+        r.render_passes(buf, c, 3)
+        r.render_passes(buf, c, 2)
+        out.append((c.load_image(), c.load_seeds()))
+    np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert np.all(out[0][0][..., 3] == 5.0)
