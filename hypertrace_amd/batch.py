@@ -127,7 +127,16 @@ CU_COUNT = 256
 # the scalar cache (same images; without bounds in LDS the register cap of
 # 4 waves per SIMD sets the occupancy).
 SCALAR_BOUNDS = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_SCALAR_BOUNDS=1"
-OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (128, 256, 512, 1024)))
+# Scalar-cache bounds, and each wave takes its own tiles and hands the
+# tile's pixels to its lanes as they finish (same images).
+WAVE_QUEUE = SCALAR_BOUNDS + " -DHT_BATCH_WAVE_QUEUE=1"
+OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (128, 256, 512, 1024)), (WAVE_QUEUE, (64, 128, 256)))
+
+
+def tile_unit(opts: Optional[str], block: int) -> int:
+    """Tiles are multiples of the block, or of one wave (64 pixels) in
+    wave-queue programs, which take tiles per wave."""
+    return 64 if opts and "HT_BATCH_WAVE_QUEUE" in opts else block
 
 
 def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mults=TILE_MULTS,
@@ -148,7 +157,7 @@ def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mu
     for opts, block in candidates:
         bprog = BatchProgram(ctx, block, fp_options=opts)
         for m in tile_mults:
-            tile = block * m
+            tile = tile_unit(opts, block) * m
             batch = Batch(ctx, bprog, specs, tile)
             for wpc in grid_waves_per_cu:
                 grid = max(1, CU_COUNT * wpc * 64 // block)

@@ -56,7 +56,7 @@ int batch_check_job(const ht_batch_job &j, size_t i) {
     if (j.type > 1) return fail(HT_ERR_OTHER, who + ": type must be 0 or 1");
     if (!j.scene || !j.canvas || !j.seeds) return fail(HT_ERR_OTHER, who + ": null buffer");
     size_t px = (size_t)j.w * j.h;
-    if (px == 0 || px > 0xFFFFFFFFull) return fail(HT_ERR_OTHER, who + ": bad image size");
+    if (px == 0 || px > 0x80000000ull) return fail(HT_ERR_OTHER, who + ": image empty or larger than 2^31 pixels");
     if (j.canvas->size < 16 * px || j.seeds->size < 4 * px)
         return fail(HT_ERR_OTHER, who + ": canvas or seed buffer smaller than the image");
     if (j.scene->size == 0) return fail(HT_ERR_OTHER, who + ": empty scene buffer");
@@ -107,9 +107,12 @@ extern "C" int ht_batch_create(ht_prog *prog, uint32_t njobs, const ht_batch_job
     int block = 0;
     HIP_TRY(hipSetDevice(prog->ctx->device));
     HIP_TRY(hipFuncGetAttribute(&block, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, prog->render_batch));
-    if (tile == 0 || tile % (uint32_t)block)
-        return fail(HT_ERR_OTHER, "ht_batch_create: tile must be a positive multiple of the program's block (" +
-                                      std::to_string(block) + ")");
+    // tiles are per block, or per wave in wave-queue programs
+    uint32_t unit = prog->batch_wave_tiles ? 64u : (uint32_t)block;
+    if (tile == 0 || tile % unit)
+        return fail(HT_ERR_OTHER, std::string("ht_batch_create: tile must be a positive multiple of ") +
+                                      (prog->batch_wave_tiles ? "64 (a wave-queue program)"
+                                                              : "the program's block (" + std::to_string(block) + ")"));
     std::vector<DevJob> table(njobs);
     uint32_t tiles = 0;
     for (uint32_t i = 0; i < njobs; ++i) {

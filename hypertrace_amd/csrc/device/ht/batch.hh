@@ -14,6 +14,12 @@
 // tile. Per pixel the work is exactly `render_passes` of that job's scene with
 // HOPS = depth, so every job's canvas and seeds are bit-identical to
 // rendering it alone.
+//
+// Program option HT_BATCH_WAVE_QUEUE (with HT_SCALAR_BOUNDS): each wave takes
+// tiles from the counter on its own (no block barrier), and inside a tile
+// each lane takes the tile's next pixel when its pixel's samples are done,
+// so a wave waits for its slowest pixel once per tile instead of once per
+// pixel round. Same canvas and seeds.
 #pragma once
 #include "render.hh"
 
@@ -38,7 +44,7 @@ template <class S>
 HD void batch_tile(const BatchJob &job, uint tile, uint tile_px) {
     const u8 *scene = reinterpret_cast<const u8 *>(job.scene);
 #ifdef HT_SCALAR_BOUNDS
-    if (threadIdx.x == 0) batch_bounds_slot() = job.bounds;
+    if (threadIdx.x % 64 == 0) batch_bounds_slot() = job.bounds;
 #else
     TopScan<typename S::Object>::prepare(S::object(scene));
 #endif
@@ -67,6 +73,110 @@ template <class S0, class S1> HD void batch_bounds_fill_impl(const BatchJob *__r
     }
 }
 
+#ifdef HT_BATCH_WAVE_QUEUE
+#ifndef HT_SCALAR_BOUNDS
+#error "HT_BATCH_WAVE_QUEUE needs HT_SCALAR_BOUNDS (LDS bounds are staged per block)"
+#endif
+// The pixels [first, end) of one job by one wave: a lane whose pixel has its
+// `spp` samples takes the next pixel of the range (one ballot for all lanes
+// that need one); each pixel's samples run in order in one lane, as in
+// render_pixel_passes.
+template <class S>
+HD void batch_wave_tile(const BatchJob &job, uint first, uint end) {
+    typedef typename S::Geo G;
+    const u8 *scene = reinterpret_cast<const u8 *>(job.scene);
+    float4 *canvas = reinterpret_cast<float4 *>(job.canvas);
+    uint *seeds = reinterpret_cast<uint *>(job.seeds);
+    const uint w = job.w, h = job.h, passes = job.spp;
+    const int hops = (int)job.depth < S::HOPS ? (int)job.depth : S::HOPS;
+    const unsigned long long below = lane_mask_below();
+    uint cursor = first;  // the same in every lane
+    bool have = false, start = true;
+    uint idx = 0, done = 0;
+    int hop = 0;
+    Context ctx;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    Light<G> light;
+    color3 emission = c3(0.0f);
+    for (;;) {
+        if (cursor < end) {
+            bool need = !have;
+            unsigned long long m = __ballot(need);
+            uint cand = cursor + (uint)__popcll(m & below);
+            if (need && cand < end) {
+                have = true;
+                idx = cand;
+                ctx.rng.state = seeds[idx];
+                ctx.hasher = hash_init();
+                acc = canvas[idx];
+                done = 0;
+                start = true;
+                if (hops <= 0 || passes == 0) {
+                    PixelSetup px = pixel_setup(idx % w, idx / w, w, h);
+                    for (uint k = 0; hops <= 0 && k < passes; ++k) {
+                        start_path<S>(scene, &ctx, px, &light, &emission);
+                        acc.w += 1.0f;
+                    }
+                    canvas[idx] = acc;
+                    seeds[idx] = ctx.rng.state;
+                    have = false;
+                }
+            }
+            cursor += (uint)__popcll(m);
+        }
+        if (__ballot(have) == 0) {
+            if (cursor >= end) break;
+            continue;
+        }
+        if (have) {
+            if (start) {
+                start_path<S>(scene, &ctx, pixel_setup(idx % w, idx / w, w, h), &light, &emission);
+                hop = 0;
+                start = false;
+            }
+            bool finished = bounce<S>(scene, &ctx, &light, &emission, hop);
+            ++hop;
+            if (hop >= hops) finished = true;
+            if (finished) {
+                acc.x += emission.x;
+                acc.y += emission.y;
+                acc.z += emission.z;
+                acc.w += 1.0f;
+                ++done;
+                start = true;
+                if (done == passes) {
+                    canvas[idx] = acc;
+                    seeds[idx] = ctx.rng.state;
+                    have = false;
+                }
+            }
+        }
+    }
+}
+
+template <class S0, class S1>
+HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *__restrict__ counter,
+                          uint total_tiles, uint tile_px) {
+    for (;;) {
+        uint t = 0;
+        if (__lane_id() == 0) t = atomicAdd(counter, 1u);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= total_tiles) return;
+        uint j = 0;
+        while (j + 1 < njobs && jobs[j + 1].tile_begin <= t) ++j;
+        const BatchJob job = jobs[j];
+        batch_bounds_slot() = job.bounds;  // this wave's slot; its own later reads see it
+        uint npx = job.w * job.h;
+        uint first = (t - job.tile_begin) * tile_px;
+        uint end = first + (npx - first < tile_px ? npx - first : tile_px);
+        if (job.type == 0) {
+            batch_wave_tile<S0>(job, first, end);
+        } else {
+            batch_wave_tile<S1>(job, first, end);
+        }
+    }
+}
+#else
 template <class S0, class S1>
 HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *__restrict__ counter,
                           uint total_tiles, uint tile_px) {
@@ -87,6 +197,7 @@ HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *_
         }
     }
 }
+#endif
 
 }  // namespace ht
 
@@ -102,8 +213,16 @@ HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *_
 #else
 #define HT_DEFINE_BATCH_BOUNDS_KERNEL(S0, S1)
 #endif
+// Wave-queue programs mark themselves: their tiles are per wave, so the
+// runtime asks only for a multiple of 64 pixels.
+#ifdef HT_BATCH_WAVE_QUEUE
+#define HT_DEFINE_BATCH_TILE_MARK extern "C" __device__ unsigned ht_batch_wave_tiles = 1;
+#else
+#define HT_DEFINE_BATCH_TILE_MARK
+#endif
 #define HT_DEFINE_BATCH_KERNEL(S0, S1)                                                                   \
     HT_DEFINE_BATCH_BOUNDS_KERNEL(S0, S1)                                                                \
+    HT_DEFINE_BATCH_TILE_MARK                                                                            \
     extern "C" __global__ void __launch_bounds__(HT_BATCH_BLOCK, HT_WAVES_PER_EU)                        \
         render_batch(const ht::BatchJob *__restrict__ jobs, unsigned njobs, unsigned *__restrict__ counter, \
                      unsigned total_tiles, unsigned tile_px) {                                           \

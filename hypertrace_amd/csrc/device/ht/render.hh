@@ -201,9 +201,9 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
                     acc = canvas[idx];
                     done = 0;
                     start = true;
-                    if (hops <= 0) {
+                    if (hops <= 0 || passes == 0) {  // no bounce loop (and nothing to do for 0 passes)
                         PixelSetup px = pixel_setup(idx % w, idx / w, w, h);
-                        for (uint k = 0; k < passes; ++k) {
+                        for (uint k = 0; hops <= 0 && k < passes; ++k) {
                             start_path<S>(scene, &ctx, px, &light, &emission);
                             acc.w += 1.0f;
                         }

@@ -128,12 +128,12 @@ HD BoundS widen(const Bound &b) {
 typedef __attribute__((address_space(4))) const BoundS ConstBoundS;
 #ifdef HT_BATCH_BLOCK
 // Batch programs: one array per job (filled by batch_bounds_fill before the
-// frame); the block publishes the address of its current job's array in
-// LDS before the tile's paths run, and the scan reads it as a wave-uniform
-// (SGPR) base.
+// frame); each wave publishes the address of its current job's array in its
+// own LDS slot before the tile's paths run, and the scan reads it as a
+// wave-uniform (SGPR) base.
 HD unsigned long long &batch_bounds_slot() {
-    __shared__ unsigned long long slot;
-    return slot;
+    __shared__ unsigned long long slot[HT_MAX_WAVES];
+    return slot[threadIdx.x / 64];
 }
 HD const ConstBoundS *scalar_bounds_base() {
     unsigned long long v = batch_bounds_slot();

@@ -47,6 +47,7 @@ struct ht_prog {
     // frame; each array holds batch_bounds_cap bounds of 64 B
     hipFunction_t batch_bounds_fill = nullptr;
     unsigned batch_bounds_cap = 0;
+    bool batch_wave_tiles = false;          // wave-queue batch program: tiles per wave (multiples of 64)
     hipFunction_t bounds_fill = nullptr;    // scalar-bounds programs: fills the bounds before each render launch
     // pixel-queue programs: resets the pixel counter before each
     // render_passes launch, which then runs as many blocks as fit at once

@@ -475,6 +475,12 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
             return fail(HT_ERR_BACKEND, "batch program has batch_bounds_fill but no readable ht_batch_bounds_cap");
         }
     }
+    if (p->render_batch) {
+        hipDeviceptr_t d = nullptr;
+        size_t bytes = 0;
+        p->batch_wave_tiles = hipModuleGetGlobal(&d, &bytes, p->module, "ht_batch_wave_tiles") == hipSuccess;
+        (void)hipGetLastError();  // absent in programs without the option
+    }
     p->wf_generate = find("wf_generate");
     p->wf_extend = find("wf_extend");
     p->wf_shade = find("wf_shade");
@@ -594,6 +600,8 @@ namespace {
 
 int check_image(const char *what, uint32_t w, uint32_t h, const ht_buf *b, size_t px_bytes) {
     if (!b) return fail(HT_ERR_OTHER, std::string(what) + ": null buffer");
+    if ((uint64_t)w * (uint64_t)h > 0x80000000ull)
+        return fail(HT_ERR_OTHER, std::string(what) + ": image larger than 2^31 pixels");
     size_t need = (size_t)w * (size_t)h * px_bytes;
     if (b->size < need)
         return fail(HT_ERR_OTHER, std::string(what) + ": buffer of " + std::to_string(b->size) + " bytes, need " + std::to_string(need));
@@ -605,7 +613,9 @@ int launch_render(ht_prog *prog, hipFunction_t fn, uint32_t w, uint32_t h, ht_bu
     if (!prog || !scene) return fail(HT_ERR_OTHER, "render: invalid argument");
     if (!fn) return fail(HT_ERR_OTHER, "render: the program has no per-scene render kernels (a batch program)");
     if (w == 0 || h == 0) return HT_OK;
-    if ((uint64_t)w * (uint64_t)h > 0xFFFFFFFFull) return fail(HT_ERR_OTHER, "render: image too large");
+    // at most 2^31 pixels (a 32 GiB canvas): pixel counters and the grid
+    // size then stay far from 32-bit wrap-around
+    if ((uint64_t)w * (uint64_t)h > 0x80000000ull) return fail(HT_ERR_OTHER, "render: image larger than 2^31 pixels");
     int rc;
     if ((rc = check_image("render canvas", w, h, canvas, 16))) return rc;
     if ((rc = check_image("render seeds", w, h, seeds, 4))) return rc;

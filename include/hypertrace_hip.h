@@ -205,7 +205,10 @@ int ht_cache_train(ht_cache *cache, uint32_t steps, uint32_t batch, float lr);
  * A batch program is built with ht_prog_build_ex from a root source that
  * instantiates HT_DEFINE_BATCH_KERNEL(Scene0, Scene1) (codegen.batch_source);
  * its block size is the kernel's launch bound; `tile` (pixels per work item,
- * a multiple of the block size) is chosen at ht_batch_create. Per pixel the
+ * a multiple of the block size; of 64 for a program built with
+ * -DHT_BATCH_WAVE_QUEUE=1, whose waves take tiles on their own and hand the
+ * tile's pixels to their lanes as they finish) is chosen at
+ * ht_batch_create. Per pixel the
  * work equals ht_render_passes of the job's scene with at most `depth` hits
  * per path, so canvases and seeds are bit-identical to rendering each job
  * alone. A batch program built with -DHT_SCALAR_BOUNDS=1 reads each job's

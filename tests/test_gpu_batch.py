@@ -6,7 +6,9 @@ import numpy as np
 import pytest
 
 from hypertrace_amd import scenes
-from hypertrace_amd.batch import SCALAR_BOUNDS, Batch, BatchProgram
+import dataclasses
+
+from hypertrace_amd.batch import SCALAR_BOUNDS, WAVE_QUEUE, Batch, BatchProgram
 from hypertrace_amd.native import HtError
 from hypertrace_amd.objects import SceneImpl
 from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
@@ -37,15 +39,19 @@ def alone(ctx, spec, frames=1):
 
 
 @pytest.mark.parametrize("block,mult,opts", [(256, 1, None), (64, 4, None), (1024, 1, None),
-                                             (256, 1, SCALAR_BOUNDS), (1024, 2, SCALAR_BOUNDS)])
+                                             (256, 1, SCALAR_BOUNDS), (1024, 2, SCALAR_BOUNDS),
+                                             (128, 1, WAVE_QUEUE), (256, 4, WAVE_QUEUE), (64, 1, WAVE_QUEUE),
+                                             (256, 0.25, WAVE_QUEUE)])
 def test_batch_jobs_identical_to_single_renders(ctx, block, mult, opts):
     """Every job's canvas and seeds equal rendering it alone, with the bounds
     staged in LDS per tile or (scalar bounds) read from the job's own FP64
-    array."""
+    array, and with each wave taking its own tiles and handing pixels to its
+    lanes as they finish (wave queue)."""
     specs = scenes.batch_jobs(**SMALL)
-    print(f"case block {block} tile {block * mult} opts {opts}", flush=True)
+    tile = int(block * mult)  # wave-queue tiles are per wave: a multiple of 64, below the block too
+    print(f"case block {block} tile {tile} opts {opts}", flush=True)
     bprog = BatchProgram(ctx, block, fp_options=opts)
-    batch = Batch(ctx, bprog, specs, block * mult)
+    batch = Batch(ctx, bprog, specs, tile)
     batch.render(grid=7, clear=False)
     ctx.sync()
     for k, spec in enumerate(specs):
@@ -55,7 +61,7 @@ def test_batch_jobs_identical_to_single_renders(ctx, block, mult, opts):
         np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
 
 
-@pytest.mark.parametrize("opts", [None, SCALAR_BOUNDS])
+@pytest.mark.parametrize("opts", [None, SCALAR_BOUNDS, WAVE_QUEUE])
 def test_graph_replays_match_eager_frames(ctx, opts):
     specs = scenes.batch_jobs(**SMALL)
     bprog = BatchProgram(ctx, 256, fp_options=opts)
@@ -78,11 +84,36 @@ def test_graph_replays_match_eager_frames(ctx, opts):
             assert np.all(a[..., 3] == specs[k].spp)  # each frame starts from a cleared canvas
 
 
+@pytest.mark.parametrize("opts", [None, WAVE_QUEUE])
+def test_batch_jobs_with_zero_spp(ctx, opts):
+    """Jobs with no samples leave their canvas and seeds as they were; the
+    other jobs are unaffected."""
+    specs = scenes.batch_jobs(**SMALL)
+    specs = [dataclasses.replace(s, spp=0) if k in (0, 3) else s for k, s in enumerate(specs)]
+    print(f"case zero spp opts {opts}", flush=True)
+    bprog = BatchProgram(ctx, 256, fp_options=opts)
+    batch = Batch(ctx, bprog, specs, 256)
+    seeds0 = [c.load_seeds().copy() for c in batch.canvases]
+    batch.render(grid=5, clear=False)
+    ctx.sync()
+    for k, spec in enumerate(specs):
+        got = batch.canvases[k]
+        if spec.spp == 0:
+            assert np.all(got.load_image() == 0.0)
+            np.testing.assert_array_equal(got.load_seeds(), seeds0[k])
+        else:
+            img, seeds = alone(ctx, spec)
+            np.testing.assert_array_equal(got.load_image().view(np.uint32), img.view(np.uint32), err_msg=f"job {k}")
+            np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
+
+
 def test_batch_argument_checks(ctx):
     specs = scenes.batch_jobs(**SMALL)[:2]
     bprog = BatchProgram(ctx, 256)
     with pytest.raises(HtError, match="tile"):
         Batch(ctx, bprog, specs, 128)  # not a multiple of the block
+    with pytest.raises(HtError, match="multiple of 64"):
+        Batch(ctx, BatchProgram(ctx, 128, fp_options=WAVE_QUEUE), specs, 96)  # per-wave tiles: multiples of 64
     wl = scenes.euclidean_spheres(16, 16, 1)
     scene_prog = Render(ctx, wl.scene).program
 
