@@ -187,6 +187,103 @@ def time_to_rmse(ctx, wl, buf, chosen, plain_render, args, reference=None):
     return res
 
 
+def same_seed_fp64_error(wl, spp, image32, fp_options=None):
+    """The FP32 image against the FP64 program's image of the same paths
+    (same seeds, same spp; C4 images): precision error, with the branch flips
+    it causes, next to the FP64 image's own Monte Carlo noise at that spp
+    (from a second FP64 image on independent seeds)."""
+    from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
+    from hypertrace_amd.types import X32_F64
+
+    W, H = wl.width, wl.height
+    with Context(0, X32_F64) as c64:
+        r64 = Render(c64, wl.scene, fp_options=fp_options)
+        b64 = EntityBuffer(c64, wl.scene)
+        imgs = []
+        for seed in (wl.seed, wl.seed ^ ORACLE_SEED_XOR):
+            c = Canvas(c64, (W, H), wl.seed_mode, seed)
+            r64.render_passes(b64, c, spp)
+            imgs.append(gamma_image(c.load_image()))
+    g32 = gamma_image(image32)
+    return {"spp": spp, "rmse_fp32_vs_fp64_same_seeds": rmse(g32, imgs[0]),
+            "max_abs": float(np.max(np.abs(g32 - imgs[0]))),
+            "pixels_within_1e-3": float(np.mean(np.all(np.abs(g32 - imgs[0]) <= 1e-3, axis=-1))),
+            "fp64_noise_rmse_at_spp": rmse(imgs[0], imgs[1]) / np.sqrt(2.0)}
+
+
+def euclidean_leg(args):
+    """The Euclidean sphere scene, 320x240, 16 passes of 1 spp, depth 4, 8
+    spheres + 1 emitter, gradient sky (the reference's CPU-runnable case):
+    one 16-pass frame of the chosen form (one render_passes launch) and of
+    the plain kernel (one launch + sync per pass, as the reference's loop),
+    time to RMSE 0.01 against the FP64 4096-spp image, and the FP32 image's
+    error against the FP64 program on the same seeds."""
+    from hypertrace_amd import scenes
+    from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
+    from hypertrace_amd.types import X32_F32
+
+    from hypertrace_amd.processing import program_options
+
+    wl = scenes.euclidean_spheres()
+    W, H, SPP = wl.width, wl.height, wl.spp
+    reference = fp64_reference(wl, args.oracle_spp)
+    with Context(0, X32_F32) as ctx:
+        buf = EntityBuffer(ctx, wl.scene)
+        chosen = Render(ctx, wl.scene)
+        plain = Render(ctx, wl.scene, fp_options=NO_CULL)
+        # the same forms with the FP32 hit window 1e-5 instead of the
+        # reference's 1e-6 (the reference's window leaves FP32 images 0.01
+        # RMSE from FP64 ones on this scene)
+        chosen_w = Render(ctx, wl.scene, fp_options=program_options(eps32=1e-5))
+        plain_w = Render(ctx, wl.scene, fp_options=program_options(eps32=1e-5, no_cull=True))
+        c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        chosen.render_passes(buf, c, SPP)  # warm
+        plain.render(buf, c)
+        times = []
+        for _ in range(3):
+            c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+            ctx.sync()
+            t0 = time.perf_counter()
+            chosen.render_passes(buf, c, SPP)
+            ctx.sync()
+            times.append(time.perf_counter() - t0)
+        t_chosen = sorted(times)[1]
+        image32 = c.load_image()
+        c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(SPP):
+            plain.render(buf, c)
+        ctx.sync()
+        t_plain = time.perf_counter() - t0
+        ttr = time_to_rmse(ctx, wl, buf, chosen, plain, args, reference)
+        ttr_w = time_to_rmse(ctx, wl, buf, chosen_w, plain_w, args, reference)
+        cw = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        chosen_w.render_passes(buf, cw, SPP)
+        image32_w = cw.load_image()
+    paths = W * H * SPP
+    keys = ("reached", "spp", "chosen_s", "plain_s", "speedup", "final_rmse", "oracle_noise_rmse")
+
+    def summary(t):
+        out = {k: t.get(k) for k in keys}
+        if not t.get("reached"):
+            out["rmse_at_max_spp"] = t["curve"][-1]  # (spp, RMSE): the FP32 image's distance floor
+        return out
+
+    return {"workload": "euclidean_spheres 320x240, 16 x 1 spp, depth 4, 8 spheres + 1 emitter (radiance 5), "
+                        "gradient sky, strict FP32",
+            "chosen": {"value": paths / t_chosen / 1e6, "unit": "Mpaths/s", "ms_per_frame": t_chosen * 1e3,
+                       "kernel": "render_passes, culled scan, one launch per frame"},
+            "plain": {"value": paths / t_plain / 1e6, "unit": "Mpaths/s", "ms_per_frame": t_plain * 1e3,
+                      "kernel": "render, element-by-element scan, one launch + sync per pass"},
+            "time_to_rmse": summary(ttr),
+            "fp64_error": same_seed_fp64_error(wl, SPP, image32),
+            "fp32_hit_window_1e-5": {
+                "options": "-DHT_EPS32=1e-05f (not the reference's results: a wider self-intersection window)",
+                "time_to_rmse": summary(ttr_w),
+                "fp64_error": same_seed_fp64_error(wl, SPP, image32_w)}}
+
+
 def cache_leg(args):
     """The radiance-cache method on the 1920x1080 hyperbolic cell scene with
     256 primitives: time until RMSE <= target against an FP64 4096-spp image,
@@ -300,6 +397,10 @@ def sdf_leg(args):
         gpu_ms = ctx.timer_end()
         wall = time.perf_counter() - t0
         img32 = gamma_image(c.load_image())
+        c16 = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+        r.render_passes(b, c16, 16)
+        image16 = c16.load_image()
+    same_seeds = same_seed_fp64_error(wl, 16, image16, fp_options=opts)
     halves = []
     t64 = time.perf_counter()
     with Context(0, X32_F64) as c64:
@@ -318,7 +419,8 @@ def sdf_leg(args):
             "fp64_reference": {"spp": 2 * (SPP // 2), "render_s": t64,
                                "rmse_fp32_vs_fp64": rmse(img32, ref),
                                "fp64_noise_rmse": rmse(gamma_image(halves[0]), gamma_image(halves[1])) / 2.0,
-                               "note": "independent seeds: the RMSE is MC noise of both images plus any FP32 bias"}}
+                               "note": "independent seeds: the RMSE is MC noise of both images plus any FP32 bias"},
+            "fp64_error": same_seeds}
 
 
 def wavefront_leg(args):
@@ -434,6 +536,7 @@ def main():
     ap.add_argument("--no-cache", action="store_true", help="skip the 1080p radiance-cache leg")
     ap.add_argument("--cache-max-spp", type=int, default=1024)
     ap.add_argument("--no-sdf", action="store_true", help="skip the 1080p SDF leg")
+    ap.add_argument("--no-euclidean", action="store_true", help="skip the 320x240 Euclidean leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batched leg")
     ap.add_argument("--no-wavefront", action="store_true", help="skip the wavefront-form leg")
     ap.add_argument("--wavefront-steps", type=int, default=2)
@@ -537,6 +640,9 @@ def main():
 
         if not args.no_rmse:
             extra["time_to_rmse"] = time_to_rmse(ctx, wl, buf, chosen, plain_render, args)
+            frame = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+            chosen.render_passes(buf, frame, SPP)
+            extra["fp64_error"] = same_seed_fp64_error(wl, SPP, frame.load_image())
 
     fl = count_flops_per_path(spec, W, H, seeds0)
     flops_per_path = fl["fp32"] + fl["fp64"]
@@ -574,6 +680,11 @@ def main():
         "roofline": roofline,
     }
     out.update(extra)
+    if not args.no_euclidean and not args.no_rmse:
+        try:
+            out["euclidean_320x240"] = euclidean_leg(args)
+        except Exception as e:
+            out["euclidean_320x240"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cache and not args.no_rmse:
         try:
             out["radiance_cache_1080p"] = cache_leg(args)

@@ -71,6 +71,8 @@ def precompile_programs(verbose: bool = False) -> None:
         extra = [(X32_F32, rr), (X32_F64, rr)] if scene.geo == "Eu" else []
         if k in (2, 5):  # the cell scene and the 1024-shape scene: the sphere-tree option
             extra.append((X32_F32, tree))
+        if k == 0:  # the Euclidean sphere scene: the wider FP32 hit window, both scans
+            extra += [(X32_F32, program_options(eps32=1e-5)), (X32_F32, program_options(eps32=1e-5, no_cull=True))]
         if k == 2:  # the cell scene: the render tuner's candidates
             extra += [(X32_F32, candidate_options(c)) for c in CANDIDATES]
             extra.append((X32_F32, program_options(scalar_bounds=True)))

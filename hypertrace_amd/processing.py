@@ -212,14 +212,18 @@ class Program:
 
 
 def program_options(fast: bool = False, rr_depth: Optional[int] = None, no_cull: bool = False,
-                    scalar_bounds: bool = False, hy_distance_fp64: bool = False) -> str:
+                    scalar_bounds: bool = False, hy_distance_fp64: bool = False,
+                    eps32: Optional[float] = None) -> str:
     """Program build options: strict (default) or fast FP32; Russian roulette
     from the rr_depth-th hit (a different method: it draws one more uniform
     per hit from there on); the element-by-element scan (comparisons); the
     culled scan's bounds in a global FP64 array read through the scalar
     cache instead of LDS (same images; frees LDS on scenes with many
     elements); the hyperbolic distance in FP64 with a cancellation-free
-    acosh (a precision method: not bit-identical to the reference)."""
+    acosh (a precision method: not bit-identical to the reference); the FP32
+    hit window `eps32` instead of the reference's 1e-6 (a precision method:
+    FP32 hit points at world scale ~10 round by ~1e-6, so the reference's
+    window lets FP32 images drift from FP64 ones; 1e-5 removes that)."""
     opts = FAST_FP if fast else STRICT_FP
     if rr_depth is not None:
         opts += f" -DHT_RR_DEPTH={int(rr_depth)}"
@@ -229,6 +233,8 @@ def program_options(fast: bool = False, rr_depth: Optional[int] = None, no_cull:
         opts += " -DHT_SCALAR_BOUNDS=1"
     if hy_distance_fp64:
         opts += " -DHT_HY_DIST_FP64=1"
+    if eps32 is not None:
+        opts += f" -DHT_EPS32={float(eps32)!r}f"
     return opts
 
 

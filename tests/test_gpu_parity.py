@@ -267,3 +267,41 @@ def test_pixel_queue_ragged_frames(ctx, size):
     np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
     np.testing.assert_array_equal(out[0][1], out[1][1])
     assert np.all(out[0][0][..., 3] == 5.0)
+
+
+def test_fp32_hit_window_option(ctx, ctx64):
+    """-DHT_EPS32=1e-5f: the culled scan equals the element-by-element scan
+    under the same window (bit for bit), and the FP32 image of the Euclidean
+    sphere scene moves to the FP64 one. With the reference's window (1e-6)
+    FP32 hit points at world scale ~8 round by about the window's size, and
+    the converged FP32 image sits ~0.01 RMSE from the FP64 image (measured on
+    this scene with the CPU oracle too); FP64 decides."""
+    wl = scenes.euclidean_spheres(96, 72)
+    print("case fp32 hit window 1e-5, euclidean spheres 96x72", flush=True)
+    spp = 256
+    buf = EntityBuffer(ctx, wl.scene)
+    imgs = {}
+    for name, fp in (("ref", None), ("wide", program_options(eps32=1e-5)),
+                     ("wide_plain", program_options(eps32=1e-5, no_cull=True))):
+        r = Render(ctx, wl.scene, fp_options=fp)
+        c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
+        r.render_passes(buf, c, spp)
+        imgs[name] = (c.load_image(), c.load_seeds())
+    np.testing.assert_array_equal(imgs["wide"][0].view(np.uint32), imgs["wide_plain"][0].view(np.uint32))
+    np.testing.assert_array_equal(imgs["wide"][1], imgs["wide_plain"][1])
+    r64 = Render(ctx64, wl.scene)
+    b64 = EntityBuffer(ctx64, wl.scene)
+    c64 = Canvas(ctx64, (wl.width, wl.height), wl.seed_mode, wl.seed)
+    r64.render_passes(b64, c64, spp)
+    ref64 = c64.load_image()
+
+    def img(c):
+        c = c.astype(np.float64)
+        return np.clip(np.maximum(c[..., :3] / c[..., 3:4], 0.0) ** (1 / 2.2), 0.0, 1.0)
+
+    def dist(a):
+        return float(np.sqrt(np.mean((img(a) - img(ref64)) ** 2)))
+
+    d_ref, d_wide = dist(imgs["ref"][0]), dist(imgs["wide"][0])
+    print(f"rmse to fp64 (same seeds, {spp} spp): reference window {d_ref:.5f}, 1e-5 window {d_wide:.5f}", flush=True)
+    assert d_wide < 0.25 * d_ref
