@@ -67,9 +67,13 @@ int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int 
  * on the device at once, after the program's pixel_queue_reset kernel, and
  * lanes take pixels from a per-program counter; launches of one program
  * are ordered on its context's stream, so they never share it); and
- * methods that do not: -DHT_RR_DEPTH=d (Russian roulette), -DHT_NO_CULL=1
- * (the element-by-element scan, same images, for comparisons). No
- * reference counterpart beyond ocl::Program build options. */
+ * methods that do not: -DHT_RR_DEPTH=d (Russian roulette),
+ * -DHT_HY_DIST_FP64=1 (hyperbolic distance in FP64, cancellation-free),
+ * -DHT_EPS32=<float> (FP32 programs: this hit window instead of the
+ * reference's 1e-6, e.g. 1e-5f for scenes at world scale ~10, where FP32
+ * roundings reach 1e-6). -DHT_NO_CULL=1 selects the element-by-element
+ * scan (same images, for comparisons). No reference counterpart beyond
+ * ocl::Program build options. */
 int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
                      const char *const *names, const char *const *contents, int double_precision,
                      int address_width, const char *fp_options, ht_prog **out);
