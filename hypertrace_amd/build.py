@@ -110,6 +110,92 @@ def precompile_batch_programs(verbose: bool = False) -> None:
         print(f"  batch and per-job programs in {time.time() - t:.1f}s")
 
 
+def _suite_jobs():
+    """(scene, config, fp options, root source or None) of every program the
+    GPU tests (tests/test_gpu_*.py, read for their workload and option
+    tables) and bench.py's legs load."""
+    import importlib.util
+
+    from . import codegen, scenes
+    from .processing import FAST_FP, NO_CULL_FP, program_options
+    from .tuning import CANDIDATES, candidate_options
+    from .types import X32_F32, X32_F64
+
+    tests_dir = os.path.join(ROOT, "tests")
+
+    def load(name):
+        path = os.path.join(tests_dir, name + ".py")
+        if not os.path.exists(path):
+            return None
+        spec = importlib.util.spec_from_file_location("_suite_" + name, path)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        return mod
+
+    jobs = []  # (scene, config, fp options, root source or None)
+    par = load("test_gpu_parity")
+    if par is not None:
+        for make in par.WORKLOADS.values():
+            s = make().scene
+            for fp in list(par.CULLED_VARIANTS.values()) + [None, NO_CULL_FP, FAST_FP, program_options(rr_depth=3)]:
+                jobs.append((s, X32_F32, fp, None))
+            jobs.append((s, X32_F64, None, None))
+        eu = scenes.euclidean_spheres(96, 72).scene
+        jobs += [(eu, X32_F32, program_options(eps32=1e-5), None),
+                 (eu, X32_F32, program_options(eps32=1e-5, no_cull=True), None)]
+    wf = load("test_gpu_wavefront")
+    if wf is not None:
+        for make, opts in wf.CASES.values():
+            s = make()[0]
+            jobs.append((s, X32_F32, opts, None))
+            jobs.append((s, X32_F32, opts, codegen.root_source(s, X32_F32, wavefront=True)))
+        s = scenes.hyperbolic_cells(80, 60).scene
+        jobs += [(s, X32_F64, None, None), (s, X32_F64, None, codegen.root_source(s, X32_F64, wavefront=True))]
+    # bench.py legs
+    cells = scenes.hyperbolic_cells().scene
+    jobs.append((cells, X32_F32, None, codegen.root_source(cells, X32_F32, wavefront=True)))
+    for wl in (scenes.hyperbolic_cells(), scenes.hyperbolic_cells(1920, 1080, 1, primitives=256)):
+        jobs += [(wl.scene, X32_F32, candidate_options(c), None) for c in CANDIDATES]
+        jobs += [(wl.scene, X32_F32, NO_CULL_FP, None), (wl.scene, X32_F64, None, None)]
+    rr = program_options(rr_depth=3)
+    sdf = scenes.sdf_shapes().scene
+    jobs += [(sdf, X32_F32, candidate_options(c, rr), None) for c in CANDIDATES]
+    jobs.append((sdf, X32_F64, rr, None))
+    return jobs
+
+
+def _suite_shard(k: int, n: int) -> int:
+    from .processing import precompile
+
+    compiled = 0
+    for scene, cfg, fp, source in _suite_jobs()[k::n]:
+        compiled += not precompile(scene, cfg, fp_options=fp, source=source)
+    return compiled
+
+
+def precompile_suite_programs(verbose: bool = False, workers: int = 8) -> None:
+    """The programs the GPU tests and bench.py's legs load, so that a GPU run
+    compiles nothing: the render tuner's candidates for the cell frames and
+    the SDF scene, the plain and FP64 programs of each leg, the wavefront
+    forms, and every scene x option pair of the parity tables. Cached
+    programs cost only their hash; missing ones compile in `workers`
+    processes (each writes the cache through its own temporary files)."""
+    t0 = time.time()
+    n = len(_suite_jobs())
+    procs = [subprocess.Popen([sys.executable, "-c",
+                               f"import sys; sys.path.insert(0, {ROOT!r}); from hypertrace_amd import build; "
+                               f"print(build._suite_shard({k}, {workers}))"],
+                              stdout=subprocess.PIPE, text=True) for k in range(workers)]
+    compiled = 0
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"precompiling the test and bench programs failed (shard exit {p.returncode})")
+        compiled += int(out.strip().splitlines()[-1])
+    if verbose:
+        print(f"  test and bench programs: {n} ({compiled} compiled) in {time.time() - t0:.1f}s")
+
+
 def build_all(verbose: bool = True) -> None:
     t = time.time()
     build_library()
@@ -117,6 +203,7 @@ def build_all(verbose: bool = True) -> None:
         print(f"built {LIB} in {time.time() - t:.1f}s")
     precompile_programs(verbose)
     precompile_batch_programs(verbose)
+    precompile_suite_programs(verbose)
 
 
 if __name__ == "__main__":
