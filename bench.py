@@ -423,13 +423,14 @@ def sdf_leg(args):
             "fp64_error": same_seeds}
 
 
-def wavefront_leg(args):
+def wavefront_leg(args, options=None):
     """The cell frame (640x480, 64 spp, 192 primitives) through the wavefront
     form (ht_render_wavefront: SoA path state in HBM, extend / material-sorted
     shade kernels) next to the megakernel's render_passes on the same
-    program options: time per frame, and whether canvas and seeds agree bit
-    for bit. The wavefront call is synchronous (it reads the live-path count
-    back every few iterations), so its time is wall-clock."""
+    program options (the main frame's tuned ones): time per frame, and
+    whether canvas and seeds agree bit for bit. The wavefront call is
+    synchronous (it reads the live-path count back every few iterations), so
+    its time is wall-clock."""
     import numpy as np
 
     from hypertrace_amd import scenes
@@ -440,8 +441,8 @@ def wavefront_leg(args):
     W, H, SPP = wl.width, wl.height, wl.spp
     with Context(0, X32_F32) as ctx:
         buf = EntityBuffer(ctx, wl.scene)
-        mega = Render(ctx, wl.scene)
-        wave = Render(ctx, wl.scene, wavefront=True)
+        mega = Render(ctx, wl.scene, fp_options=options)
+        wave = Render(ctx, wl.scene, fp_options=options, wavefront=True)
         results = {}
         for name, run in (("megakernel", lambda c: mega.render_passes(buf, c, SPP)),
                           ("wavefront", lambda c: wave.render_wavefront(buf, c, SPP))):
@@ -465,7 +466,7 @@ def wavefront_leg(args):
     for r in results.values():
         del r["img"], r["seeds"]
     return {"workload": f"hyperbolic_cells {W}x{H}, {SPP} spp, 192 primitives (strict FP32, culled scan)",
-            "megakernel": results["megakernel"], "wavefront": results["wavefront"],
+            "options": options, "megakernel": results["megakernel"], "wavefront": results["wavefront"],
             "bit_identical": same,
             "wavefront_over_megakernel": results["wavefront"]["ms_per_frame"] / results["megakernel"]["ms_per_frame"]}
 
@@ -697,7 +698,7 @@ def main():
             out["batch_32_jobs"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_wavefront:
         try:
-            out["wavefront_cells"] = wavefront_leg(args)
+            out["wavefront_cells"] = wavefront_leg(args, out["render_tuning"]["best"]["options"])
         except Exception as e:
             out["wavefront_cells"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_sdf:

@@ -153,7 +153,9 @@ def _suite_jobs():
         jobs += [(s, X32_F64, None, None), (s, X32_F64, None, codegen.root_source(s, X32_F64, wavefront=True))]
     # bench.py legs
     cells = scenes.hyperbolic_cells().scene
-    jobs.append((cells, X32_F32, None, codegen.root_source(cells, X32_F32, wavefront=True)))
+    wf_src = codegen.root_source(cells, X32_F32, wavefront=True)
+    jobs.append((cells, X32_F32, None, wf_src))
+    jobs += [(cells, X32_F32, candidate_options(c), wf_src) for c in CANDIDATES]  # the wavefront leg's program
     for wl in (scenes.hyperbolic_cells(), scenes.hyperbolic_cells(1920, 1080, 1, primitives=256)):
         jobs += [(wl.scene, X32_F32, candidate_options(c), None) for c in CANDIDATES]
         jobs += [(wl.scene, X32_F32, NO_CULL_FP, None), (wl.scene, X32_F64, None, None)]
