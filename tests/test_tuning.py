@@ -1,4 +1,6 @@
-"""Host logic of the render tuner: candidate option strings (no GPU)."""
+"""Host logic of the render tuner and the program options: option strings,
+the batch tile unit, and that every option macro is one the device sources
+read (no GPU)."""
 import pathlib
 import re
 
@@ -30,3 +32,36 @@ def test_knob_macros_are_read_by_the_device_sources():
     assert {"HT_CULL_UNROLL", "HT_WAVES_PER_EU", "HT_SCALAR_BOUNDS", "HT_CULL_GROUPS", "HT_PIXEL_QUEUE"} <= names
     for name in names:
         assert re.search(rf"#\s*(ifdef|ifndef|if|elif)\b[^\n]*\b{name}\b", source), name
+
+
+def test_program_option_strings():
+    from hypertrace_amd.processing import NO_CULL_FP, program_options
+
+    assert program_options() == STRICT_FP
+    assert program_options(no_cull=True) == NO_CULL_FP
+    assert program_options(eps32=1e-5).endswith("-DHT_EPS32=1e-05f")
+    assert "-DHT_HY_DIST_FP64=1" in program_options(hy_distance_fp64=True)
+    assert "-DHT_RR_DEPTH=3" in program_options(rr_depth=3)
+
+
+def test_batch_tile_unit():
+    from hypertrace_amd.batch import OPTIONS, SCALAR_BOUNDS, WAVE_QUEUE, tile_unit
+
+    assert tile_unit(None, 256) == 256
+    assert tile_unit(SCALAR_BOUNDS, 512) == 512
+    assert tile_unit(WAVE_QUEUE, 256) == 64
+    assert any(opts == WAVE_QUEUE for opts, _ in OPTIONS)
+
+
+def test_option_macros_are_read_by_the_device_sources():
+    from hypertrace_amd.batch import WAVE_QUEUE
+    from hypertrace_amd.processing import program_options
+
+    source = "\n".join(p.read_text() for p in DEVICE.glob("*.hh"))
+    opts = " ".join([program_options(rr_depth=3, no_cull=True, scalar_bounds=True, hy_distance_fp64=True,
+                                     eps32=1e-5), WAVE_QUEUE])
+    names = set(re.findall(r"-D(HT_[A-Z0-9_]+)=", opts))
+    assert {"HT_EPS32", "HT_BATCH_WAVE_QUEUE", "HT_HY_DIST_FP64", "HT_RR_DEPTH", "HT_NO_CULL"} <= names
+    for name in names:
+        assert re.search(rf"\b{name}\b", source) and re.search(rf"#\s*(ifdef|ifndef|if|elif)\b[^\n]*\b{name}\b",
+                                                                 source), name
