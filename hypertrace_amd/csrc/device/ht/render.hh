@@ -273,7 +273,14 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 // HT_SCALAR_BOUNDS programs also export bounds_fill(scene), which the
 // runtime launches before each render / render_passes / render_cached
 // launch of the program.
-#ifdef HT_SCALAR_BOUNDS
+#if defined(HT_SCALAR_BOUNDS) && defined(HT_CULL_GROUPS)
+// the tree is built by one block in its LDS (the launch's other blocks exit)
+#define HT_DEFINE_BOUNDS_KERNEL(S)                                                                          \
+    extern "C" __global__ void __launch_bounds__(256) bounds_fill(const unsigned char *__restrict__ scene) { \
+        if (blockIdx.x == 0)                                                                                \
+            ht::TopScan<typename S::Object>::fill_scalar_block(S::object(scene), ht::ht_scalar_bounds);     \
+    }
+#elif defined(HT_SCALAR_BOUNDS)
 #define HT_DEFINE_BOUNDS_KERNEL(S)                                                                          \
     extern "C" __global__ void __launch_bounds__(256) bounds_fill(const unsigned char *__restrict__ scene) { \
         ht::TopScan<typename S::Object>::fill_scalar(S::object(scene), ht::ht_scalar_bounds,                \

@@ -145,7 +145,13 @@ HD const ConstBoundS *scalar_bounds_base() {
 // Scene programs: the top-level vector's bounds, written by the program's
 // bounds_fill kernel before each render launch (launches of one program on
 // one stream are ordered, so one array per loaded program).
+// With HT_CULL_GROUPS the array holds the tree's leaves in Morton order
+// followed by the group nodes (fill_scalar_block).
+#ifdef HT_CULL_GROUPS
+__device__ BoundS ht_scalar_bounds[HT_BOUNDS_CAP + HT_BOUNDS_CAP / HT_CULL_GROUPS];
+#else
 __device__ BoundS ht_scalar_bounds[HT_BOUNDS_CAP];
+#endif
 HD const ConstBoundS *scalar_bounds_base() { return (const ConstBoundS *)ht_scalar_bounds; }
 #endif
 // One 64-byte scalar load per bound (s_load_dwordx16), whatever fields the
@@ -458,13 +464,20 @@ template <class G> HD WaveScratch<G> *wave_scratch_lds() {
 // is a dependent chain (node load -> FP64 test -> next node) at low
 // occupancy, and nodes holding a horoball cannot use the "behind" cull. The
 // flat scan is the default.
-#if (defined(HT_CULL_TREE) || defined(HT_CULL_GROUPS)) && defined(HT_SCALAR_BOUNDS)
-#error "HT_CULL_TREE / HT_CULL_GROUPS walk the LDS bounds; HT_SCALAR_BOUNDS has none"
+#if defined(HT_CULL_TREE) && defined(HT_SCALAR_BOUNDS)
+#error "HT_CULL_TREE walks the LDS bounds; HT_SCALAR_BOUNDS has none"
+#endif
+#if defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS) && defined(HT_BATCH_BLOCK)
+#error "batch programs keep one flat bounds array per job: HT_CULL_GROUPS needs the LDS bounds there"
 #endif
 // Program option HT_CULL_GROUPS = g (2..16, a power of two): the same tree,
 // scanned in two levels with every lane of a wave in step: the nodes that
 // cover g consecutive leaves are tested first, and a node's leaves only when
-// some lane's ray passes it (a wave-uniform skip).
+// some lane's ray passes it (a wave-uniform skip). With HT_SCALAR_BOUNDS the
+// bounds_fill kernel builds the tree in its own LDS once per launch and
+// writes the leaves (in Morton order, each with its element index) and the
+// group nodes to the global FP64 array, which the scan reads through the
+// scalar cache: no LDS in the render kernels.
 #ifdef HT_CULL_GROUPS
 static constexpr uint CULL_GROUP = HT_CULL_GROUPS;
 static_assert(CULL_GROUP >= 2 && CULL_GROUP <= 16 && (CULL_GROUP & (CULL_GROUP - 1)) == 0, "group size");
@@ -572,6 +585,7 @@ HD unsigned long long hit_key(float d, usize i) {
 template <class T> struct TopScan {
     static HD void prepare(const u8 *) {}
     static HD void fill_scalar(const u8 *, BoundS *, uint, uint) {}
+    static HD void fill_scalar_block(const u8 *, BoundS *) {}
     template <class G>
     static HD real detect(const u8 *self, Context *ctx, typename T::Cache *cache, Light<G> *light) {
         return T::detect(self, ctx, cache, light);
@@ -625,6 +639,40 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             (void)self, (void)out, (void)first, (void)stride;
         }
     }
+
+#if defined(HT_SCALAR_BOUNDS) && defined(HT_CULL_GROUPS)
+    // All threads of one block: the bounds of the two-level scalar-load scan.
+    // With the tree (TREE_OK and at least TREE_MIN_ELEMENTS elements) `out`
+    // gets leaf j of the tree at j (element index in `pad`, NO_ELEMENT for
+    // padding) and group node FIRST + k at P + k; otherwise element i's bound
+    // at i, as fill_scalar writes them (detect takes the same decision).
+    static HD void fill_scalar_block(const u8 *self, BoundS *out) {
+        if constexpr (CULL && TREE_OK) {
+            usize n = V::len(self);
+            if (n > (usize)BOUNDS_CAP) return;
+            Bound *b = bounds_lds<OV>();
+            for (uint i = threadIdx.x; i < (uint)n; i += blockDim.x) b[i] = element_bound<E, typename E::Map>(V::elem(self, i));
+            __syncthreads();
+            if (n < TREE_MIN_ELEMENTS) {
+                for (uint i = threadIdx.x; i < (uint)n; i += blockDim.x) out[i] = widen(b[i]);
+                return;
+            }
+            build_tree((uint)n, b);
+            constexpr uint P = BOUNDS_CAP, FIRST = P / CULL_GROUP;
+            const TreeLds *t = tree_lds<OV>();
+            for (uint j = threadIdx.x; j < P; j += blockDim.x) {
+                unsigned short e = t->order[j];
+                BoundS w = widen(e == NO_ELEMENT ? no_bound() : b[e]);
+                w.pad = __uint_as_float((uint)e);
+                out[j] = w;
+            }
+            for (uint k = FIRST + threadIdx.x; k < 2 * FIRST; k += blockDim.x)
+                out[P + (k - FIRST)] = widen(MakeBound<Bound>::node(t->node[k], t->horo[k] != 0));
+        } else {
+            fill_scalar(self, out, threadIdx.x, blockDim.x);
+        }
+    }
+#endif
 
     // All threads of the block (see TreeLds).
     static HD void build_tree(uint n, const Bound *b) {
@@ -800,7 +848,31 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         }
     }
 
-#ifdef HT_CULL_GROUPS
+#if defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS)
+    // walk_groups over the global array fill_scalar_block writes: every
+    // node and leaf bound is one scalar load.
+    template <class CR, class Push, class Drain>
+    static HD void walk_groups_scalar(const CR &cr, const ConstBoundS *sb, Push &push, Drain &drain) {
+        constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP, FIRST = P / GS;
+        for (uint k = 0; k < FIRST; ++k) {
+            BoundS q = load_bound(sb + P + k);
+            if (q.r == -2.0) continue;  // no elements under this node
+            bool pass = cr.candidate(q);
+            if (__ballot(pass) == 0) continue;
+            bool c[GS];
+            uint el[GS];
+#pragma unroll
+            for (uint j = 0; j < GS; ++j) {
+                BoundS lb = load_bound(sb + k * GS + j);
+                el[j] = __float_as_uint(lb.pad);
+                c[j] = el[j] != NO_ELEMENT && pass && cr.candidate(lb);
+            }
+#pragma unroll
+            for (uint j = 0; j < GS; ++j) push(el[j], c[j]);
+            drain();
+        }
+    }
+#elif defined(HT_CULL_GROUPS)
     // The group nodes of the tree in order, each tested by every lane; a
     // node's CULL_GROUP leaves are tested (and their pairs queued) only when
     // some lane's ray passes the node. HT_CULL_GROUPS only.
@@ -882,7 +954,9 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             bool tree = false;
             if constexpr (TREE_OK) tree = n >= TREE_MIN_ELEMENTS;
             if (tree) {
-#ifdef HT_CULL_GROUPS
+#if defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS)
+                if constexpr (TREE_OK) walk_groups_scalar(cr, sb, push, drain);
+#elif defined(HT_CULL_GROUPS)
                 if constexpr (TREE_OK) walk_groups(cr, push, drain);
 #else
                 if constexpr (TREE_OK) walk_tree(cr, append);

@@ -8,7 +8,9 @@ against resident waves; HT_SCALAR_BOUNDS moves the element bounds from LDS
 (FP32, widened per lane) to a global FP64 array read through the scalar
 cache, which frees the LDS that limits occupancy on scenes with many
 elements; HT_CULL_GROUPS scans groups of Morton-ordered elements first and
-skips a group's elements when no lane of the wave passes its sphere;
+skips a group's elements when no lane of the wave passes its sphere (with
+scalar bounds the bounds_fill kernel writes the tree's leaves and group
+spheres to the global array, so the render kernels hold no tree in LDS);
 HT_PIXEL_QUEUE launches only as many blocks as the device holds at once and
 has each lane take its next pixel from a global counter when its pixel's
 samples are done, which removes the tail of waves whose slowest pixel holds
@@ -31,7 +33,8 @@ from .processing import STRICT_FP, Canvas, Context, EntityBuffer, Render
 CANDIDATES: Tuple[Tuple[int, int, bool, int, bool], ...] = (
     (4, 4, False, 0, False), (4, 5, False, 0, False), (2, 5, False, 0, False), (4, 6, False, 0, False),
     (4, 4, True, 0, False), (4, 5, True, 0, False), (4, 6, True, 0, False), (4, 5, False, 4, False),
-    (4, 4, False, 0, True), (4, 4, True, 0, True), (4, 6, True, 0, True))
+    (4, 4, False, 0, True), (4, 4, True, 0, True), (4, 6, True, 0, True),
+    (4, 4, True, 4, False), (4, 6, True, 4, False), (4, 6, True, 8, False), (4, 6, True, 4, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,

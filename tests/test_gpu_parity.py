@@ -217,6 +217,8 @@ CULLED_VARIANTS = {
     "groups8_w5": STRICT_FP + " -DHT_CULL_GROUPS=8 -DHT_WAVES_PER_EU=5",
     "pixel_queue": STRICT_FP + " -DHT_PIXEL_QUEUE=1",
     "pixel_queue_sb_w6": STRICT_FP + " -DHT_PIXEL_QUEUE=1 -DHT_SCALAR_BOUNDS=1 -DHT_WAVES_PER_EU=6",
+    "sb_groups4": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4",
+    "sb_groups8_w6_q": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=8 -DHT_WAVES_PER_EU=6 -DHT_PIXEL_QUEUE=1",
 }
 
 
@@ -228,7 +230,10 @@ CULLED_VARIANTS = {
     ("hy_example", False, "scalar_bounds"), ("sdf_1024", False, "scalar_bounds"),
     ("hy_cells", False, "scalar_bounds_u2_w5"), ("hy_cells", True, "groups4"), ("sdf_1024", False, "groups4"),
     ("hy_cells", False, "groups8_w5"), ("hy_cells", True, "pixel_queue"), ("eu_ball", False, "pixel_queue"),
-    ("sdf_1024", False, "pixel_queue_sb_w6"), ("hy_example", False, "pixel_queue_sb_w6")])
+    ("sdf_1024", False, "pixel_queue_sb_w6"), ("hy_example", False, "pixel_queue_sb_w6"),
+    ("hy_cells", True, "sb_groups4"), ("sdf_1024", False, "sb_groups4"), ("eu_ball", False, "sb_groups4"),
+    ("hy_cells", False, "sb_groups8_w6_q"), ("sdf_1024", False, "sb_groups8_w6_q"),
+    ("hy_example", False, "sb_groups8_w6_q")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
