@@ -66,11 +66,20 @@ template <class S0, class S1> HD void batch_bounds_fill_impl(const BatchJob *__r
     const BatchJob job = jobs[blockIdx.x];
     const u8 *scene = reinterpret_cast<const u8 *>(job.scene);
     BoundS *out = reinterpret_cast<BoundS *>(job.bounds);
+#ifdef HT_CULL_GROUPS
+    // the job's tree in this block's LDS, then its leaves and group nodes
+    if (job.type == 0) {
+        TopScan<typename S0::Object>::fill_scalar_block(S0::object(scene), out);
+    } else {
+        TopScan<typename S1::Object>::fill_scalar_block(S1::object(scene), out);
+    }
+#else
     if (job.type == 0) {
         TopScan<typename S0::Object>::fill_scalar(S0::object(scene), out, threadIdx.x, blockDim.x);
     } else {
         TopScan<typename S1::Object>::fill_scalar(S1::object(scene), out, threadIdx.x, blockDim.x);
     }
+#endif
 }
 
 #ifdef HT_BATCH_WAVE_QUEUE
@@ -202,10 +211,17 @@ HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *_
 }  // namespace ht
 
 // HT_SCALAR_BOUNDS batch programs also export batch_bounds_fill(jobs,
-// njobs) (one block per job) and the capacity of a job's bounds array.
+// njobs) (one block of 256 threads per job) and the number of 64-byte
+// entries of a job's bounds array (with HT_CULL_GROUPS the tree's leaves
+// and then its group nodes).
+#ifdef HT_CULL_GROUPS
+#define HT_BATCH_BOUND_ENTRIES (HT_BOUNDS_CAP + HT_BOUNDS_CAP / HT_CULL_GROUPS)
+#else
+#define HT_BATCH_BOUND_ENTRIES HT_BOUNDS_CAP
+#endif
 #ifdef HT_SCALAR_BOUNDS
 #define HT_DEFINE_BATCH_BOUNDS_KERNEL(S0, S1)                                                            \
-    extern "C" __device__ unsigned ht_batch_bounds_cap = HT_BOUNDS_CAP;                                  \
+    extern "C" __device__ unsigned ht_batch_bounds_cap = HT_BATCH_BOUND_ENTRIES;                         \
     extern "C" __global__ void __launch_bounds__(256)                                                    \
         batch_bounds_fill(const ht::BatchJob *__restrict__ jobs, unsigned njobs) {                       \
         ht::batch_bounds_fill_impl<S0, S1>(jobs, njobs);                                                 \

@@ -467,9 +467,6 @@ template <class G> HD WaveScratch<G> *wave_scratch_lds() {
 #if defined(HT_CULL_TREE) && defined(HT_SCALAR_BOUNDS)
 #error "HT_CULL_TREE walks the LDS bounds; HT_SCALAR_BOUNDS has none"
 #endif
-#if defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS) && defined(HT_BATCH_BLOCK)
-#error "batch programs keep one flat bounds array per job: HT_CULL_GROUPS needs the LDS bounds there"
-#endif
 // Program option HT_CULL_GROUPS = g (2..16, a power of two): the same tree,
 // scanned in two levels with every lane of a wave in step: the nodes that
 // cover g consecutive leaves are tested first, and a node's leaves only when
@@ -477,7 +474,8 @@ template <class G> HD WaveScratch<G> *wave_scratch_lds() {
 // bounds_fill kernel builds the tree in its own LDS once per launch and
 // writes the leaves (in Morton order, each with its element index) and the
 // group nodes to the global FP64 array, which the scan reads through the
-// scalar cache: no LDS in the render kernels.
+// scalar cache: no LDS in the render kernels (batch programs: one such
+// array per job, filled by batch_bounds_fill, one block per job).
 #ifdef HT_CULL_GROUPS
 static constexpr uint CULL_GROUP = HT_CULL_GROUPS;
 static_assert(CULL_GROUP >= 2 && CULL_GROUP <= 16 && (CULL_GROUP & (CULL_GROUP - 1)) == 0, "group size");

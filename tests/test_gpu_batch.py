@@ -16,6 +16,7 @@ from hypertrace_amd.types import X32_F32
 
 pytestmark = pytest.mark.gpu
 
+GROUPS4 = " -DHT_CULL_GROUPS=4"
 SMALL = dict(n=8, min_size=(24, 16), max_size=(96, 72), spp_range=(1, 4), prim_range=(8, 64), depths=(4, 7, 12))
 
 
@@ -41,7 +42,8 @@ def alone(ctx, spec, frames=1):
 @pytest.mark.parametrize("block,mult,opts", [(256, 1, None), (64, 4, None), (1024, 1, None),
                                              (256, 1, SCALAR_BOUNDS), (1024, 2, SCALAR_BOUNDS),
                                              (128, 1, WAVE_QUEUE), (256, 4, WAVE_QUEUE), (64, 1, WAVE_QUEUE),
-                                             (256, 0.25, WAVE_QUEUE)])
+                                             (256, 0.25, WAVE_QUEUE), (256, 1, SCALAR_BOUNDS + GROUPS4),
+                                             (128, 0.5, WAVE_QUEUE + GROUPS4), (64, 2, WAVE_QUEUE + " -DHT_CULL_GROUPS=8")])
 def test_batch_jobs_identical_to_single_renders(ctx, block, mult, opts):
     """Every job's canvas and seeds equal rendering it alone, with the bounds
     staged in LDS per tile or (scalar bounds) read from the job's own FP64
@@ -61,7 +63,7 @@ def test_batch_jobs_identical_to_single_renders(ctx, block, mult, opts):
         np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
 
 
-@pytest.mark.parametrize("opts", [None, SCALAR_BOUNDS, WAVE_QUEUE])
+@pytest.mark.parametrize("opts", [None, SCALAR_BOUNDS, WAVE_QUEUE, WAVE_QUEUE + GROUPS4])
 def test_graph_replays_match_eager_frames(ctx, opts):
     specs = scenes.batch_jobs(**SMALL)
     bprog = BatchProgram(ctx, 256, fp_options=opts)
