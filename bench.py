@@ -632,6 +632,19 @@ def main():
             extra["plain"] = {"kernel": "render: one launch + sync per pass, element-by-element scan, strict FP32",
                               "value": W * H * SPP / plain_wall / 1e6, "unit": "Mpaths/s",
                               "ms_per_frame": plain_wall * 1e3, "gpu_ms_per_frame": plain_gpu_ms}
+            # the reference's loop unchanged (one synchronous render per pass)
+            # over the chosen program: what a drop-in caller gets
+            canvas_d = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+            chosen.render(buf, canvas_d)  # warm
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(SPP):
+                chosen.render(buf, canvas_d)
+            dropin_wall = time.perf_counter() - t0
+            extra["dropin_per_pass"] = {"kernel": "render: one launch + sync per pass on the chosen program "
+                                                  "(the reference's loop, culled scan)",
+                                        "value": W * H * SPP / dropin_wall / 1e6, "unit": "Mpaths/s",
+                                        "ms_per_frame": dropin_wall * 1e3}
             fast = Render(ctx, wl.scene, fp_options=FAST_FP)
             fwall, fgpu = time_chosen(fast, args.steps, args.warmup)
             extra["fast_fp"] = {"kernel": "render_passes, culled scan, fast FP32 (FMA contraction, reciprocal "
