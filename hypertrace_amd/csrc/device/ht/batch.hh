@@ -92,7 +92,6 @@ template <class S0, class S1> HD void batch_bounds_fill_impl(const BatchJob *__r
 // render_pixel_passes.
 template <class S>
 HD void batch_wave_tile(const BatchJob &job, uint first, uint end) {
-    typedef typename S::Geo G;
     const u8 *scene = reinterpret_cast<const u8 *>(job.scene);
     float4 *canvas = reinterpret_cast<float4 *>(job.canvas);
     uint *seeds = reinterpret_cast<uint *>(job.seeds);
@@ -100,66 +99,20 @@ HD void batch_wave_tile(const BatchJob &job, uint first, uint end) {
     const int hops = (int)job.depth < S::HOPS ? (int)job.depth : S::HOPS;
     const unsigned long long below = lane_mask_below();
     uint cursor = first;  // the same in every lane
-    bool have = false, start = true;
-    uint idx = 0, done = 0;
-    int hop = 0;
-    Context ctx;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    Light<G> light;
-    color3 emission = c3(0.0f);
+    QueuedPixel<S> p;
     for (;;) {
         if (cursor < end) {
-            bool need = !have;
+            bool need = !p.have;
             unsigned long long m = __ballot(need);
             uint cand = cursor + (uint)__popcll(m & below);
-            if (need && cand < end) {
-                have = true;
-                idx = cand;
-                ctx.rng.state = seeds[idx];
-                ctx.hasher = hash_init();
-                acc = canvas[idx];
-                done = 0;
-                start = true;
-                if (hops <= 0 || passes == 0) {
-                    PixelSetup px = pixel_setup(idx % w, idx / w, w, h);
-                    for (uint k = 0; hops <= 0 && k < passes; ++k) {
-                        start_path<S>(scene, &ctx, px, &light, &emission);
-                        acc.w += 1.0f;
-                    }
-                    canvas[idx] = acc;
-                    seeds[idx] = ctx.rng.state;
-                    have = false;
-                }
-            }
+            if (need && cand < end) p.take(cand, w, h, scene, canvas, seeds, passes, hops);
             cursor += (uint)__popcll(m);
         }
-        if (__ballot(have) == 0) {
+        if (__ballot(p.have) == 0) {
             if (cursor >= end) break;
             continue;
         }
-        if (have) {
-            if (start) {
-                start_path<S>(scene, &ctx, pixel_setup(idx % w, idx / w, w, h), &light, &emission);
-                hop = 0;
-                start = false;
-            }
-            bool finished = bounce<S>(scene, &ctx, &light, &emission, hop);
-            ++hop;
-            if (hop >= hops) finished = true;
-            if (finished) {
-                acc.x += emission.x;
-                acc.y += emission.y;
-                acc.z += emission.z;
-                acc.w += 1.0f;
-                ++done;
-                start = true;
-                if (done == passes) {
-                    canvas[idx] = acc;
-                    seeds[idx] = ctx.rng.state;
-                    have = false;
-                }
-            }
-        }
+        if (p.have) p.step(w, h, scene, canvas, seeds, passes, hops);
     }
 }
 

@@ -156,6 +156,74 @@ HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ sce
     seeds[idx] = ctx.rng.state;
 }
 
+// A lane's pixel in the queued forms (program options HT_PIXEL_QUEUE and
+// HT_BATCH_WAVE_QUEUE): render_pixel_passes's state kept across the rounds
+// of a loop in which every round advances each busy lane's path by one hit,
+// so a lane can finish its pixel and take another while the others go on.
+// The samples of a pixel run in order in one lane, so canvas and seeds are
+// render_pixel_passes's bit for bit.
+template <class S> struct QueuedPixel {
+    typedef typename S::Geo G;
+    Context ctx;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    Light<G> light;
+    color3 emission = c3(0.0f);
+    uint idx = 0, done = 0;
+    int hop = 0;
+    bool have = false, start = true;
+
+    // Takes pixel i; a pixel with no bounce loop (hops <= 0) or no passes
+    // is finished at once.
+    HD void take(uint i, uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
+                 uint *__restrict__ seeds, uint passes, int hops) {
+        have = true;
+        idx = i;
+        ctx.rng.state = seeds[i];
+        ctx.hasher = hash_init();
+        acc = canvas[i];
+        done = 0;
+        start = true;
+        if (hops <= 0 || passes == 0) {
+            PixelSetup px = pixel_setup(i % w, i / w, w, h);
+            for (uint k = 0; hops <= 0 && k < passes; ++k) {
+                start_path<S>(scene, &ctx, px, &light, &emission);
+                acc.w += 1.0f;
+            }
+            canvas[i] = acc;
+            seeds[i] = ctx.rng.state;
+            have = false;
+        }
+    }
+
+    // One hit of the current sample (starting the sample first if needed);
+    // after the pixel's last sample the pixel is written back.
+    HD void step(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas, uint *__restrict__ seeds,
+                 uint passes, int hops) {
+        if (start) {
+            // the pixel position is recomputed per sample: fewer live registers
+            start_path<S>(scene, &ctx, pixel_setup(idx % w, idx / w, w, h), &light, &emission);
+            hop = 0;
+            start = false;
+        }
+        bool end = bounce<S>(scene, &ctx, &light, &emission, hop);
+        ++hop;
+        if (hop >= hops) end = true;
+        if (end) {
+            acc.x += emission.x;
+            acc.y += emission.y;
+            acc.z += emission.z;
+            acc.w += 1.0f;
+            ++done;
+            start = true;
+            if (done == passes) {
+                canvas[idx] = acc;
+                seeds[idx] = ctx.rng.state;
+                have = false;
+            }
+        }
+    }
+};
+
 #ifdef HT_PIXEL_QUEUE
 // Program option HT_PIXEL_QUEUE: render_passes as a persistent kernel (the
 // runtime launches only the blocks that fit on the device at once, after
@@ -171,20 +239,13 @@ __device__ uint ht_pixel_next;
 template <class S>
 HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
                              uint *__restrict__ seeds, uint passes) {
-    typedef typename S::Geo G;
     const uint n = w * h;
-    const int hops = S::HOPS;
     const unsigned long long below = lane_mask_below();
-    bool have = false, drained = false, start = true;
-    uint idx = 0, done = 0;
-    int hop = 0;
-    Context ctx;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    Light<G> light;
-    color3 emission = c3(0.0f);
+    QueuedPixel<S> p;
+    bool drained = false;
     for (;;) {
         if (!drained) {
-            bool need = !have;
+            bool need = !p.have;
             unsigned long long m = __ballot(need);
             if (m != 0) {
                 uint leader = (uint)__ffsll((long long)m) - 1u;
@@ -193,55 +254,14 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
                 base = __shfl(base, (int)leader);
                 if (base + (uint)__popcll(m) >= n) drained = true;  // the counter only grows
                 uint cand = base + (uint)__popcll(m & below);
-                if (need && cand < n) {
-                    have = true;
-                    idx = cand;
-                    ctx.rng.state = seeds[idx];
-                    ctx.hasher = hash_init();
-                    acc = canvas[idx];
-                    done = 0;
-                    start = true;
-                    if (hops <= 0 || passes == 0) {  // no bounce loop (and nothing to do for 0 passes)
-                        PixelSetup px = pixel_setup(idx % w, idx / w, w, h);
-                        for (uint k = 0; hops <= 0 && k < passes; ++k) {
-                            start_path<S>(scene, &ctx, px, &light, &emission);
-                            acc.w += 1.0f;
-                        }
-                        canvas[idx] = acc;
-                        seeds[idx] = ctx.rng.state;
-                        have = false;
-                    }
-                }
+                if (need && cand < n) p.take(cand, w, h, scene, canvas, seeds, passes, S::HOPS);
             }
         }
-        if (__ballot(have) == 0) {
+        if (__ballot(p.have) == 0) {
             if (drained) break;
             continue;
         }
-        if (have) {
-            if (start) {
-                // the pixel position is recomputed per sample: fewer live registers
-                start_path<S>(scene, &ctx, pixel_setup(idx % w, idx / w, w, h), &light, &emission);
-                hop = 0;
-                start = false;
-            }
-            bool end = bounce<S>(scene, &ctx, &light, &emission, hop);
-            ++hop;
-            if (hop >= hops) end = true;
-            if (end) {
-                acc.x += emission.x;
-                acc.y += emission.y;
-                acc.z += emission.z;
-                acc.w += 1.0f;
-                ++done;
-                start = true;
-                if (done == passes) {
-                    canvas[idx] = acc;
-                    seeds[idx] = ctx.rng.state;
-                    have = false;
-                }
-            }
-        }
+        if (p.have) p.step(w, h, scene, canvas, seeds, passes, S::HOPS);
     }
 }
 #endif
