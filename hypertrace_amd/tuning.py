@@ -34,7 +34,8 @@ CANDIDATES: Tuple[Tuple[int, int, bool, int, bool], ...] = (
     (4, 4, False, 0, False), (4, 5, False, 0, False), (2, 5, False, 0, False), (4, 6, False, 0, False),
     (4, 4, True, 0, False), (4, 5, True, 0, False), (4, 6, True, 0, False), (4, 5, False, 4, False),
     (4, 4, False, 0, True), (4, 4, True, 0, True), (4, 6, True, 0, True),
-    (4, 4, True, 4, False), (4, 6, True, 4, False), (4, 6, True, 8, False), (4, 6, True, 4, True))
+    (4, 4, True, 4, False), (4, 6, True, 4, False), (4, 6, True, 8, False), (4, 6, True, 4, True),
+    (4, 4, True, 4, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
