@@ -140,6 +140,9 @@ def _suite_jobs():
             for fp in list(par.CULLED_VARIANTS.values()) + [None, NO_CULL_FP, FAST_FP, program_options(rr_depth=3)]:
                 jobs.append((s, X32_F32, fp, None))
             jobs.append((s, X32_F64, None, None))
+        cap = par.CAPACITY_BASE().scene
+        jobs += [(cap, X32_F32, par.CULLED_VARIANTS[v], None) for v in par.CAPACITY_VARIANTS]
+        jobs.append((cap, X32_F32, NO_CULL_FP, None))
         eu = scenes.euclidean_spheres(96, 72).scene
         jobs += [(eu, X32_F32, program_options(eps32=1e-5), None),
                  (eu, X32_F32, program_options(eps32=1e-5, no_cull=True), None)]

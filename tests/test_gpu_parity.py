@@ -310,3 +310,40 @@ def test_fp32_hit_window_option(ctx, ctx64):
     d_ref, d_wide = dist(imgs["ref"][0]), dist(imgs["wide"][0])
     print(f"rmse to fp64 (same seeds, {spp} spp): reference window {d_ref:.5f}, 1e-5 window {d_wide:.5f}", flush=True)
     assert d_wide < 0.25 * d_ref
+
+
+# A cell scene whose program has bounds capacity 64, and the variants run on it.
+CAPACITY_BASE = lambda: scenes.hyperbolic_cells(64, 48, primitives=48)  # noqa: E731
+CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue"]
+
+
+@pytest.mark.parametrize("variant", CAPACITY_VARIANTS)
+def test_element_counts_across_the_bounds_capacity(ctx, variant):
+    """A program built for 48 primitives (bounds capacity 64) renders cell
+    scenes of the same type with 0, 3, 12, 16, 48, 64 and 120 primitives: an
+    empty vector, scenes below and at the tree's 16-element threshold (flat
+    layout, then the tree with groups), and more elements than the capacity
+    (the unculled scan). Each equals the element-by-element scan bit for
+    bit."""
+    from hypertrace_amd.objects import ObjectVec, SceneImpl
+
+    base = CAPACITY_BASE().scene
+    culled = Render(ctx, base, fp_options=CULLED_VARIANTS[variant])
+    plain = Render(ctx, base, fp_options=NO_CULL)
+    for n in (0, 3, 12, 16, 48, 64, 120):
+        print(f"case {variant}: {n} primitives", flush=True)
+        if n == 0:
+            scene = SceneImpl(base.geo, base.view, ObjectVec([], proto=base.object.items[0]), base.background,
+                              base.hops)
+        else:
+            scene = scenes.hyperbolic_cells(64, 48, primitives=n).scene
+        buf = EntityBuffer(ctx, scene)
+        out = []
+        for r in (culled, plain):
+            c = Canvas(ctx, (64, 48), 1, 11)
+            r.render_passes(buf, c, 3)
+            r.render(buf, c)
+            out.append((c.load_image(), c.load_seeds()))
+        np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32), err_msg=f"{n} primitives")
+        np.testing.assert_array_equal(out[0][1], out[1][1], err_msg=f"{n} primitives")
+        assert np.all(out[0][0][..., 3] == 4.0)
