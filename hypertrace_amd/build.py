@@ -111,7 +111,7 @@ def precompile_batch_programs(verbose: bool = False) -> None:
 
 
 def _suite_jobs():
-    """(scene, config, fp options, root source or None) of every program the
+    """(scene, config, fp options, root source or None[, root name]) of every program the
     GPU tests (tests/test_gpu_*.py, read for their workload and option
     tables) and bench.py's legs load."""
     import importlib.util
@@ -146,6 +146,20 @@ def _suite_jobs():
         eu = scenes.euclidean_spheres(96, 72).scene
         jobs += [(eu, X32_F32, program_options(eps32=1e-5), None),
                  (eu, X32_F32, program_options(eps32=1e-5, no_cull=True), None)]
+    bt = load("test_gpu_batch")
+    if bt is not None:
+        from .batch import batch_root
+        from .objects import SceneImpl
+
+        programs = {(block, opts) for block, _, opts in bt.IDENTITY_CASES}
+        programs |= {(256, opts) for opts in bt.GRAPH_OPTS} | {(128, None), (256, None)}
+        programs |= {(block, opts) for _, opts, block in bt.EDGE_SETS}
+        jobs += [(None, X32_F32, opts, batch_root(X32_F32, block), "batch.hip")
+                 for block, opts in sorted(programs, key=str)]
+        for prims, _, _ in bt.EDGE_SETS:
+            for spec in bt.edge_specs(prims):  # the single-job programs the jobs are compared with
+                sc = spec.scene
+                jobs.append((SceneImpl(sc.geo, sc.view, sc.object, sc.background, spec.depth), X32_F32, None, None))
     wf = load("test_gpu_wavefront")
     if wf is not None:
         for make, opts in wf.CASES.values():
@@ -173,8 +187,10 @@ def _suite_shard(k: int, n: int) -> int:
     from .processing import precompile
 
     compiled = 0
-    for scene, cfg, fp, source in _suite_jobs()[k::n]:
-        compiled += not precompile(scene, cfg, fp_options=fp, source=source)
+    for job in _suite_jobs()[k::n]:
+        scene, cfg, fp, source = job[:4]
+        name = job[4] if len(job) > 4 else "render.hip"
+        compiled += not precompile(scene, cfg, name=name, fp_options=fp, source=source)
     return compiled
 
 

@@ -39,11 +39,15 @@ def alone(ctx, spec, frames=1):
     return c.load_image(), c.load_seeds()
 
 
-@pytest.mark.parametrize("block,mult,opts", [(256, 1, None), (64, 4, None), (1024, 1, None),
-                                             (256, 1, SCALAR_BOUNDS), (1024, 2, SCALAR_BOUNDS),
-                                             (128, 1, WAVE_QUEUE), (256, 4, WAVE_QUEUE), (64, 1, WAVE_QUEUE),
-                                             (256, 0.25, WAVE_QUEUE), (256, 1, SCALAR_BOUNDS + GROUPS4),
-                                             (128, 0.5, WAVE_QUEUE + GROUPS4), (64, 2, WAVE_QUEUE + " -DHT_CULL_GROUPS=8")])
+# (block, tile / block, program options) of the identity test
+IDENTITY_CASES = [(256, 1, None), (64, 4, None), (1024, 1, None), (256, 1, SCALAR_BOUNDS), (1024, 2, SCALAR_BOUNDS),
+                  (128, 1, WAVE_QUEUE), (256, 4, WAVE_QUEUE), (64, 1, WAVE_QUEUE), (256, 0.25, WAVE_QUEUE),
+                  (256, 1, SCALAR_BOUNDS + GROUPS4), (128, 0.5, WAVE_QUEUE + GROUPS4),
+                  (64, 2, WAVE_QUEUE + " -DHT_CULL_GROUPS=8")]
+GRAPH_OPTS = [None, SCALAR_BOUNDS, WAVE_QUEUE, WAVE_QUEUE + GROUPS4]  # 256-thread programs
+
+
+@pytest.mark.parametrize("block,mult,opts", IDENTITY_CASES)
 def test_batch_jobs_identical_to_single_renders(ctx, block, mult, opts):
     """Every job's canvas and seeds equal rendering it alone, with the bounds
     staged in LDS per tile or (scalar bounds) read from the job's own FP64
@@ -63,7 +67,7 @@ def test_batch_jobs_identical_to_single_renders(ctx, block, mult, opts):
         np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
 
 
-@pytest.mark.parametrize("opts", [None, SCALAR_BOUNDS, WAVE_QUEUE, WAVE_QUEUE + GROUPS4])
+@pytest.mark.parametrize("opts", GRAPH_OPTS)
 def test_graph_replays_match_eager_frames(ctx, opts):
     specs = scenes.batch_jobs(**SMALL)
     bprog = BatchProgram(ctx, 256, fp_options=opts)
@@ -107,6 +111,33 @@ def test_batch_jobs_with_zero_spp(ctx, opts):
             img, seeds = alone(ctx, spec)
             np.testing.assert_array_equal(got.load_image().view(np.uint32), img.view(np.uint32), err_msg=f"job {k}")
             np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
+
+
+# Job sets at the edges of the culled scan: more elements than the batch
+# program's capacity (256: the unculled scan), across the tree's 16-element
+# threshold, and up to the capacity; (primitive range, program options, block).
+EDGE_SETS = [((257, 300), None, 256), ((257, 300), WAVE_QUEUE + GROUPS4, 128), ((12, 20), WAVE_QUEUE + GROUPS4, 128),
+             ((250, 256), SCALAR_BOUNDS + GROUPS4, 256)]
+
+
+def edge_specs(prims):
+    return scenes.batch_jobs(n=4, min_size=(24, 16), max_size=(48, 32), spp_range=(1, 2), prim_range=prims,
+                             depths=(4, 7))
+
+
+@pytest.mark.parametrize("prims,opts,block", EDGE_SETS)
+def test_batch_element_counts_at_the_edges(ctx, prims, opts, block):
+    specs = edge_specs(prims)
+    print(f"case primitives {prims} opts {opts} block {block}", flush=True)
+    bprog = BatchProgram(ctx, block, fp_options=opts)
+    batch = Batch(ctx, bprog, specs, block)
+    batch.render(grid=6, clear=False)
+    ctx.sync()
+    for k, spec in enumerate(specs):
+        img, seeds = alone(ctx, spec)
+        np.testing.assert_array_equal(batch.canvases[k].load_image().view(np.uint32), img.view(np.uint32),
+                                      err_msg=f"job {k}")
+        np.testing.assert_array_equal(batch.canvases[k].load_seeds(), seeds, err_msg=f"job {k}")
 
 
 def test_batch_argument_checks(ctx):
