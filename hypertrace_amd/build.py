@@ -140,6 +140,8 @@ def _suite_jobs():
             for fp in list(par.CULLED_VARIANTS.values()) + [None, NO_CULL_FP, FAST_FP, program_options(rr_depth=3)]:
                 jobs.append((s, X32_F32, fp, None))
             jobs.append((s, X32_F64, None, None))
+        jobs += [(par.WORKLOADS[name]().scene, X32_F32, program_options(hy_distance_fp64=True), None)
+                 for name in par.HY_DIST_FP64_CASES]
         cap = par.CAPACITY_BASE().scene
         jobs += [(cap, X32_F32, par.CULLED_VARIANTS[v], None) for v in par.CAPACITY_VARIANTS]
         jobs.append((cap, X32_F32, NO_CULL_FP, None))

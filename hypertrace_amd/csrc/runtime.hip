@@ -395,6 +395,11 @@ int get_code_object(const char *root_name, const char *root_src, int n_headers, 
         g_cache_hit = 1;
         return HT_OK;
     }
+    if (const char *log_compiles = std::getenv("HT_LOG_COMPILES")) {
+        // diagnostics: which programs a run compiles instead of loading them from the cache
+        if (*log_compiles) std::fprintf(stderr, "ht: hipRTC compile of %s (%s, options: %s)\n", name.c_str(), key,
+                                        fp_options ? fp_options : kDefaultFpOptions);
+    }
     hiprtcProgram prog;
     hiprtcResult r = hiprtcCreateProgram(&prog, root_src, name.c_str(), n_headers, contents, names);
     if (r != HIPRTC_SUCCESS) return fail(HT_ERR_BACKEND, std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r));

@@ -46,7 +46,9 @@ int ht_ctx_sync(ht_ctx *ctx);
  * mirrors ADDRESS_WIDTH (32 or 64). Compiled code objects are cached on disk
  * under $HT_CACHE_DIR (default: the library's directory + "/../_cache"),
  * keyed by a hash of sources and options. If $HT_DUMP_SOURCE is set, the root
- * source is written to ./render.hip (render.rs:74-80 wrote ./render.ocl). */
+ * source is written to ./render.hip (render.rs:74-80 wrote ./render.ocl); if
+ * $HT_LOG_COMPILES is set, every build that misses the cache says so on
+ * stderr. */
 int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int n_headers,
                   const char *const *names, const char *const *contents, int double_precision,
                   int address_width, ht_prog **out);

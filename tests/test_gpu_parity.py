@@ -85,7 +85,10 @@ def test_single_pass_parity_f64(ctx64, name):
     assert ok.mean() >= 0.999, f"{name}: only {ok.mean():.5f} of pixels within 1e-4"
 
 
-@pytest.mark.parametrize("name", ["hy_cells", "hy_example"])
+HY_DIST_FP64_CASES = ["hy_cells", "hy_example"]
+
+
+@pytest.mark.parametrize("name", HY_DIST_FP64_CASES)
 def test_hy_distance_fp64_option(ctx, name):
     """Program option HT_HY_DIST_FP64 (the hyperbolic distance in FP64 with a
     cancellation-free acosh; a precision method, not the drop-in). Against
