@@ -555,8 +555,8 @@ def main():
                          "the known-size copy that scales FETCH_SIZE / WRITE_SIZE counter readings")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the render pass shards by "
-              "pixel rows, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the render pass would shard by "
+              "pixel rows, one row band per GPU, with per-pixel results unchanged)", file=sys.stderr)
         sys.exit(2)
 
     from hypertrace_amd import scenes

@@ -1,6 +1,6 @@
 """Build steps: the HIP runtime library (hipcc, gfx950) and the ahead-of-time
 compiled render programs of the standard workloads (hipRTC into the
-code-object cache, so a fresh GPU box does not compile them again)."""
+code-object cache, so a run on a fresh machine compiles none of them)."""
 from __future__ import annotations
 
 import os

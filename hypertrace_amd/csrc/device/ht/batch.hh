@@ -18,8 +18,8 @@
 // Program option HT_BATCH_WAVE_QUEUE (with HT_SCALAR_BOUNDS): each wave takes
 // tiles from the counter on its own (no block barrier), and inside a tile
 // each lane takes the tile's next pixel when its pixel's samples are done,
-// so a wave waits for its slowest pixel once per tile instead of once per
-// pixel round. Same canvas and seeds.
+// so a wave waits for its slowest pixel once per tile instead of once for
+// every pixel each lane renders in turn. Same canvas and seeds.
 #pragma once
 #include "render.hh"
 

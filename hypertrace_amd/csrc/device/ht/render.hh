@@ -157,9 +157,10 @@ HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ sce
 }
 
 // A lane's pixel in the queued forms (program options HT_PIXEL_QUEUE and
-// HT_BATCH_WAVE_QUEUE): render_pixel_passes's state kept across the rounds
-// of a loop in which every round advances each busy lane's path by one hit,
-// so a lane can finish its pixel and take another while the others go on.
+// HT_BATCH_WAVE_QUEUE): render_pixel_passes's state kept across the
+// iterations of a loop in which each iteration advances every busy lane's
+// path by one hit, so a lane can finish its pixel and take another while the
+// others go on.
 // The samples of a pixel run in order in one lane, so canvas and seeds are
 // render_pixel_passes's bit for bit.
 template <class S> struct QueuedPixel {

@@ -15,7 +15,7 @@ HT_PIXEL_QUEUE launches only as many blocks as the device holds at once and
 has each lane take its next pixel from a global counter when its pixel's
 samples are done, which removes the tail of waves whose slowest pixel holds
 the other 63 lanes (frames with more pixels than resident lanes gain; a
-frame with fewer pixels than two rounds of resident lanes loses). Every
+frame with fewer than twice as many pixels as resident lanes loses). Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer

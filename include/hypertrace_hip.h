@@ -155,7 +155,7 @@ int ht_mo_compose(ht_ctx *ctx, ht_buf *scene, int real_size, uint32_t n, const u
 
 /* ---- Learned radiance cache. No reference counterpart (the reference has no
  * learned part); these extend the render pass with a cached method whose
- * error is judged against the FP64 program.
+ * error is measured against the FP64 program.
  *
  * MLP: 12 features (position, direction, normal, albedo), each encoded as
  * sin / cos of 2^b pi x for b < 6 (144 inputs), 5 hidden layers of 64 with
