@@ -65,3 +65,23 @@ def test_option_macros_are_read_by_the_device_sources():
     for name in names:
         assert re.search(rf"\b{name}\b", source) and re.search(rf"#\s*(ifdef|ifndef|if|elif)\b[^\n]*\b{name}\b",
                                                                  source), name
+
+
+def test_bench_fixed_knobs_and_gpu_count_guard():
+    """bench.py's --render-knobs parser gives the tuner's option strings, and
+    --gpus above 1 exits with a message (one GPU only)."""
+    import importlib.util
+    import subprocess
+    import sys
+
+    root = pathlib.Path(__file__).resolve().parents[1]
+    spec = importlib.util.spec_from_file_location("_bench", root / "bench.py")
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    k = bench.fixed_knobs("4,6,sb,g4,q")["best"]
+    assert (k["unroll"], k["waves_per_eu"], k["scalar_bounds"], k["groups"], k["pixel_queue"]) == (4, 6, True, 4, True)
+    assert k["options"] == candidate_options((4, 6, True, 4, True))
+    assert bench.fixed_knobs("2,5")["best"]["options"] == options(2, 5)
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU only" in r.stderr
