@@ -350,7 +350,7 @@ def cache_leg(args):
 
 
 def fixed_knobs(text, base=None):
-    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q]" (no timing)."""
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p]" (no timing)."""
     from hypertrace_amd.processing import STRICT_FP
     from hypertrace_amd.tuning import options
     knobs = text.split(",")
@@ -358,10 +358,11 @@ def fixed_knobs(text, base=None):
     sb = "sb" in knobs[2:]
     groups = next((int(k[1:]) for k in knobs[2:] if k.startswith("g")), 0)
     queue = "q" in knobs[2:]
+    prio = "p" in knobs[2:]
     return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                     "pixel_queue": queue,
+                     "pixel_queue": queue, "wave_prio": prio,
                      "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups,
-                                        pixel_queue=queue)},
+                                        pixel_queue=queue, wave_prio=prio)},
             "table": [], "fixed": True}
 
 
@@ -545,10 +546,10 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
-    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q]",
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p]",
                     help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
                          "instead of tuning (keeps a profiled trace to one program)")
-    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q]",
+    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p]",
                     help="the same for the SDF leg's program")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "

@@ -132,6 +132,14 @@ HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ sce
         int hop = 0;
         uint done = 0;
         bool start = true;
+#ifdef HT_WAVE_PRIO
+        // Program option HT_WAVE_PRIO: the wave's issue priority steps down
+        // from 3 as its slowest lane completes each quarter of the passes, so
+        // the waves with the most work left (the frame's critical path) are
+        // issued first on their SIMD. Scheduling only: same results.
+        uint level = 3;
+        __builtin_amdgcn_s_setprio(3);
+#endif
         for (;;) {
             if (start) {
                 if (done == passes) break;
@@ -150,6 +158,14 @@ HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ sce
                 ++done;
                 start = true;
             }
+#ifdef HT_WAVE_PRIO
+            if (level > 0 && __ballot(done < passes * (4 - level) / 4) == 0) {
+                --level;
+                if (level == 2) __builtin_amdgcn_s_setprio(2);
+                else if (level == 1) __builtin_amdgcn_s_setprio(1);
+                else __builtin_amdgcn_s_setprio(0);
+            }
+#endif
         }
     }
     canvas[idx] = acc;

@@ -15,7 +15,10 @@ HT_PIXEL_QUEUE launches only as many blocks as the device holds at once and
 has each lane take its next pixel from a global counter when its pixel's
 samples are done, which removes the tail of waves whose slowest pixel holds
 the other 63 lanes (frames with more pixels than resident lanes gain; a
-frame with fewer than twice as many pixels as resident lanes loses). Every
+frame with fewer than twice as many pixels as resident lanes loses);
+HT_WAVE_PRIO steps each wave's issue priority down as its slowest lane
+completes each quarter of the passes, so the waves with the most work left
+(the frame's critical path) issue first. Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -29,17 +32,18 @@ from typing import Dict, Optional, Sequence, Tuple
 
 from .processing import STRICT_FP, Canvas, Context, EntityBuffer, Render
 
-# (cull unroll, waves per SIMD, scalar bounds, group size or 0, pixel queue)
-CANDIDATES: Tuple[Tuple[int, int, bool, int, bool], ...] = (
+# (cull unroll, waves per SIMD, scalar bounds, group size or 0, pixel queue, wave priority)
+CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, False, 0, False), (4, 5, False, 0, False), (2, 5, False, 0, False), (4, 6, False, 0, False),
     (4, 4, True, 0, False), (4, 5, True, 0, False), (4, 6, True, 0, False), (4, 5, False, 4, False),
     (4, 4, False, 0, True), (4, 4, True, 0, True), (4, 6, True, 0, True),
     (4, 4, True, 4, False), (4, 6, True, 4, False), (4, 6, True, 8, False), (4, 6, True, 4, True),
-    (4, 4, True, 4, True))
+    (4, 4, True, 4, True),
+    (4, 4, True, 0, False, True), (4, 5, True, 0, False, True), (4, 6, True, 4, False, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
-            pixel_queue: bool = False) -> str:
+            pixel_queue: bool = False, wave_prio: bool = False) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
@@ -47,28 +51,30 @@ def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool 
         opts += f" -DHT_CULL_GROUPS={int(groups)}"
     if pixel_queue:
         opts += " -DHT_PIXEL_QUEUE=1"
+    if wave_prio:
+        opts += " -DHT_WAVE_PRIO=1"
     return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool]:
     unroll, waves = int(candidate[0]), int(candidate[1])
     sb = bool(candidate[2]) if len(candidate) > 2 else False
     groups = int(candidate[3]) if len(candidate) > 3 else 0
     queue = bool(candidate[4]) if len(candidate) > 4 else False
-    return unroll, waves, sb, groups, queue
+    prio = bool(candidate[5]) if len(candidate) > 5 else False
+    return unroll, waves, sb, groups, queue, prio
 
 
 def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
     """Program options of one tuner candidate tuple."""
-    unroll, waves, sb, groups, queue = _knobs(candidate)
-    return options(unroll, waves, base, sb, groups, queue)
+    return options(*_knobs(candidate)[:2], base, *_knobs(candidate)[2:])
 
 
 def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mode: int = 1, seed: int = 0,
                 candidates: Sequence[Sequence] = CANDIDATES, reps: int = 3, base: str = STRICT_FP,
                 cache_path: Optional[str] = None) -> Dict:
     """Times `spp` passes of render_passes (HIP events, median of `reps`) for
-    each (unroll, waves[, scalar bounds[, groups[, pixel queue]]]) candidate; returns {"best": {...},
+    each (unroll, waves[, scalar bounds[, groups[, pixel queue[, wave priority]]]]) candidate; returns {"best": {...},
     "table": [...]} with the program options of the best."""
     key = f"gfx950:{scene.cpp()}:{width}x{height}:{spp}:{base}"
     if cache_path and os.path.exists(cache_path):
@@ -79,8 +85,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb, groups, queue = _knobs(cand)
-        opts = options(unroll, waves, base, sb, groups, queue)
+        unroll, waves, sb, groups, queue, prio = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups, queue, prio)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -91,7 +97,7 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             r.render_passes(buf, c, spp)
             times.append(ctx.timer_end())
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                      "pixel_queue": queue, "ms": sorted(times)[len(times) // 2], "options": opts})
+                      "pixel_queue": queue, "wave_prio": prio, "ms": sorted(times)[len(times) // 2], "options": opts})
     best = min(table, key=lambda t: t["ms"])
     result = {"best": best, "table": table}
     if cache_path:

@@ -222,6 +222,8 @@ CULLED_VARIANTS = {
     "pixel_queue_sb_w6": STRICT_FP + " -DHT_PIXEL_QUEUE=1 -DHT_SCALAR_BOUNDS=1 -DHT_WAVES_PER_EU=6",
     "sb_groups4": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4",
     "sb_groups8_w6_q": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=8 -DHT_WAVES_PER_EU=6 -DHT_PIXEL_QUEUE=1",
+    "sb_prio_w5": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_WAVES_PER_EU=5 -DHT_WAVE_PRIO=1",
+    "prio": STRICT_FP + " -DHT_WAVE_PRIO=1",
 }
 
 
@@ -236,7 +238,8 @@ CULLED_VARIANTS = {
     ("sdf_1024", False, "pixel_queue_sb_w6"), ("hy_example", False, "pixel_queue_sb_w6"),
     ("hy_cells", True, "sb_groups4"), ("sdf_1024", False, "sb_groups4"), ("eu_ball", False, "sb_groups4"),
     ("hy_cells", False, "sb_groups8_w6_q"), ("sdf_1024", False, "sb_groups8_w6_q"),
-    ("hy_example", False, "sb_groups8_w6_q")])
+    ("hy_example", False, "sb_groups8_w6_q"), ("hy_cells", True, "sb_prio_w5"), ("sdf_1024", False, "sb_prio_w5"),
+    ("eu_ball", False, "prio"), ("hy_example", False, "prio")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give

@@ -17,8 +17,9 @@ def test_candidates_distinct():
 
 
 def test_knob_defaults():
-    assert _knobs((4, 5)) == (4, 5, False, 0, False)
+    assert _knobs((4, 5)) == (4, 5, False, 0, False, False)
     assert options(4, 5) == candidate_options((4, 5, False, 0, False))
+    assert candidate_options((4, 5, True, 0, False, True)).endswith("-DHT_SCALAR_BOUNDS=1 -DHT_WAVE_PRIO=1")
     assert "-DHT_PIXEL_QUEUE=1" in options(4, 4, pixel_queue=True)
     assert "-DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=8" in options(2, 6, scalar_bounds=True, groups=8)
 
@@ -29,7 +30,8 @@ def test_knob_macros_are_read_by_the_device_sources():
     names = set()
     for c in CANDIDATES:
         names |= set(re.findall(r"-D(HT_[A-Z0-9_]+)=", candidate_options(c)))
-    assert {"HT_CULL_UNROLL", "HT_WAVES_PER_EU", "HT_SCALAR_BOUNDS", "HT_CULL_GROUPS", "HT_PIXEL_QUEUE"} <= names
+    assert {"HT_CULL_UNROLL", "HT_WAVES_PER_EU", "HT_SCALAR_BOUNDS", "HT_CULL_GROUPS", "HT_PIXEL_QUEUE",
+            "HT_WAVE_PRIO"} <= names
     for name in names:
         assert re.search(rf"#\s*(ifdef|ifndef|if|elif)\b[^\n]*\b{name}\b", source), name
 
@@ -81,6 +83,7 @@ def test_bench_fixed_knobs_and_gpu_count_guard():
     k = bench.fixed_knobs("4,6,sb,g4,q")["best"]
     assert (k["unroll"], k["waves_per_eu"], k["scalar_bounds"], k["groups"], k["pixel_queue"]) == (4, 6, True, 4, True)
     assert k["options"] == candidate_options((4, 6, True, 4, True))
+    assert bench.fixed_knobs("4,5,sb,p")["best"]["options"] == candidate_options((4, 5, True, 0, False, True))
     assert bench.fixed_knobs("2,5")["best"]["options"] == options(2, 5)
     r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
