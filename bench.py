@@ -45,7 +45,7 @@ FP32_NONPACKED_PEAK = 78.6  # TFLOP/s without packed FP32 (this project compiles
 # HBM bytes per render_passes launch on this workload, from the FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by the calibration copy
 # (profiles/render_passes_hyperbolic_cells_culled.md).
-MEASURED_TRAFFIC_BYTES = (6910 + 6000) * 1024
+MEASURED_TRAFFIC_BYTES = (6914 + 6000) * 1024
 NO_CULL = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_NO_CULL=1"
 ORACLE_SEED_XOR = 0x0BAD5EED
 
