@@ -133,8 +133,11 @@ WAVE_QUEUE = SCALAR_BOUNDS + " -DHT_BATCH_WAVE_QUEUE=1"
 # The wave queue with each job's bounds scanned in two levels (groups of 4
 # Morton-ordered elements; the per-job trees are built by batch_bounds_fill).
 WAVE_QUEUE_GROUPS = WAVE_QUEUE + " -DHT_CULL_GROUPS=4"
+# Block tiles with each wave's issue priority stepped down as its pixels'
+# samples complete (the waves with the most work left issue first).
+SCALAR_BOUNDS_PRIO = SCALAR_BOUNDS + " -DHT_WAVE_PRIO=1"
 OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (128, 256, 512, 1024)), (WAVE_QUEUE, (64, 128, 256)),
-           (WAVE_QUEUE_GROUPS, (64, 128, 256)))
+           (WAVE_QUEUE_GROUPS, (64, 128, 256)), (SCALAR_BOUNDS_PRIO, (128, 256)))
 
 
 def tile_unit(opts: Optional[str], block: int) -> int:

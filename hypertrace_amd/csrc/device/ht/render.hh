@@ -260,7 +260,29 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
     const unsigned long long below = lane_mask_below();
     QueuedPixel<S> p;
     bool drained = false;
+#ifdef HT_WAVE_PRIO
+    // Once the counter has passed the frame the waves still busy are the
+    // frame's tail: each sets its issue priority by the work its slowest lane
+    // has left (3: more than three quarters of the passes ... 0: less than a
+    // quarter), so the longest remaining work issues first.
+    uint level = 0;
+#endif
     for (;;) {
+#ifdef HT_WAVE_PRIO
+        if (drained) {
+            uint want = 0;
+            if (__ballot(p.have && p.done < passes / 4) != 0) want = 3;
+            else if (__ballot(p.have && p.done < passes / 2) != 0) want = 2;
+            else if (__ballot(p.have && p.done < passes - passes / 4) != 0) want = 1;
+            if (want != level) {
+                level = want;
+                if (want == 3) __builtin_amdgcn_s_setprio(3);
+                else if (want == 2) __builtin_amdgcn_s_setprio(2);
+                else if (want == 1) __builtin_amdgcn_s_setprio(1);
+                else __builtin_amdgcn_s_setprio(0);
+            }
+        }
+#endif
         if (!drained) {
             bool need = !p.have;
             unsigned long long m = __ballot(need);

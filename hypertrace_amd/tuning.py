@@ -18,7 +18,8 @@ the other 63 lanes (frames with more pixels than resident lanes gain; a
 frame with fewer than twice as many pixels as resident lanes loses);
 HT_WAVE_PRIO steps each wave's issue priority down as its slowest lane
 completes each quarter of the passes, so the waves with the most work left
-(the frame's critical path) issue first. Every
+(the frame's critical path) issue first (with the pixel queue: once the
+queue is empty, by the work each wave's current pixels have left). Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -39,7 +40,8 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, False, 0, True), (4, 4, True, 0, True), (4, 6, True, 0, True),
     (4, 4, True, 4, False), (4, 6, True, 4, False), (4, 6, True, 8, False), (4, 6, True, 4, True),
     (4, 4, True, 4, True),
-    (4, 4, True, 0, False, True), (4, 5, True, 0, False, True), (4, 6, True, 4, False, True))
+    (4, 4, True, 0, False, True), (4, 5, True, 0, False, True), (4, 6, True, 4, False, True),
+    (4, 4, True, 0, True, True), (4, 4, True, 4, True, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,

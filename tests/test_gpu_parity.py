@@ -224,6 +224,8 @@ CULLED_VARIANTS = {
     "sb_groups8_w6_q": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=8 -DHT_WAVES_PER_EU=6 -DHT_PIXEL_QUEUE=1",
     "sb_prio_w5": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_WAVES_PER_EU=5 -DHT_WAVE_PRIO=1",
     "prio": STRICT_FP + " -DHT_WAVE_PRIO=1",
+    "sb_q_prio": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_PIXEL_QUEUE=1 -DHT_WAVE_PRIO=1",
+    "sb_g4_q_prio": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_PIXEL_QUEUE=1 -DHT_WAVE_PRIO=1",
 }
 
 
@@ -239,7 +241,8 @@ CULLED_VARIANTS = {
     ("hy_cells", True, "sb_groups4"), ("sdf_1024", False, "sb_groups4"), ("eu_ball", False, "sb_groups4"),
     ("hy_cells", False, "sb_groups8_w6_q"), ("sdf_1024", False, "sb_groups8_w6_q"),
     ("hy_example", False, "sb_groups8_w6_q"), ("hy_cells", True, "sb_prio_w5"), ("sdf_1024", False, "sb_prio_w5"),
-    ("eu_ball", False, "prio"), ("hy_example", False, "prio")])
+    ("eu_ball", False, "prio"), ("hy_example", False, "prio"), ("hy_cells", True, "sb_q_prio"),
+    ("sdf_1024", False, "sb_g4_q_prio"), ("hy_cells", False, "sb_g4_q_prio")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
@@ -259,17 +262,18 @@ def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio"])
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
-def test_pixel_queue_ragged_frames(ctx, size):
+def test_pixel_queue_ragged_frames(ctx, size, variant):
     """The pixel-queue program on frames smaller than a block or with a
     partial last wave, launched twice (the counter is reset before every
     launch): the same canvas and seeds as the one-thread-per-pixel kernel."""
     w, h = size
     wl = scenes.euclidean_spheres(w, h)
-    print(f"case pixel queue {w}x{h}", flush=True)
+    print(f"case {variant} {w}x{h}", flush=True)
     buf = EntityBuffer(ctx, wl.scene)
     out = []
-    for fp in (CULLED_VARIANTS["pixel_queue"], None):
+    for fp in (CULLED_VARIANTS[variant], None):
         r = Render(ctx, wl.scene, fp_options=fp)
         c = Canvas(ctx, (w, h), wl.seed_mode, wl.seed)
         r.render_passes(buf, c, 3)
@@ -320,7 +324,7 @@ def test_fp32_hit_window_option(ctx, ctx64):
 
 # A cell scene whose program has bounds capacity 64, and the variants run on it.
 CAPACITY_BASE = lambda: scenes.hyperbolic_cells(64, 48, primitives=48)  # noqa: E731
-CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue"]
+CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue", "sb_g4_q_prio"]
 
 
 @pytest.mark.parametrize("variant", CAPACITY_VARIANTS)
