@@ -68,7 +68,9 @@ int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int 
  * -DHT_PIXEL_QUEUE=1 (ht_render_passes launches only the blocks that fit
  * on the device at once, after the program's pixel_queue_reset kernel, and
  * lanes take pixels from a per-program counter; launches of one program
- * are ordered on its context's stream, so they never share it); and
+ * are ordered on its context's stream, so they never share it),
+ * -DHT_WAVE_PRIO=1 (each wave's issue priority steps down as its pixels'
+ * samples complete, so the waves with the most work left issue first); and
  * methods that do not: -DHT_RR_DEPTH=d (Russian roulette),
  * -DHT_HY_DIST_FP64=1 (hyperbolic distance in FP64, cancellation-free),
  * -DHT_EPS32=<float> (FP32 programs: this hit window instead of the
