@@ -130,14 +130,17 @@ SCALAR_BOUNDS = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_SCALAR_BOUNDS=1"
 # Scalar-cache bounds, and each wave takes its own tiles and hands the
 # tile's pixels to its lanes as they finish (same images).
 WAVE_QUEUE = SCALAR_BOUNDS + " -DHT_BATCH_WAVE_QUEUE=1"
+# Wave priority (HT_WAVE_PRIO): block tiles step each wave's issue priority
+# down as its pixels' samples complete; the wave queue raises it on the
+# frame's last tiles by the work the wave's tile has left. Either way the
+# waves that will finish last issue first.
+SCALAR_BOUNDS_PRIO = SCALAR_BOUNDS + " -DHT_WAVE_PRIO=1"
+WAVE_QUEUE_PRIO = WAVE_QUEUE + " -DHT_WAVE_PRIO=1"
 # The wave queue with each job's bounds scanned in two levels (groups of 4
 # Morton-ordered elements; the per-job trees are built by batch_bounds_fill).
-WAVE_QUEUE_GROUPS = WAVE_QUEUE + " -DHT_CULL_GROUPS=4"
-# Block tiles with each wave's issue priority stepped down as its pixels'
-# samples complete (the waves with the most work left issue first).
-SCALAR_BOUNDS_PRIO = SCALAR_BOUNDS + " -DHT_WAVE_PRIO=1"
+WAVE_QUEUE_GROUPS = WAVE_QUEUE_PRIO + " -DHT_CULL_GROUPS=4"
 OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (128, 256, 512, 1024)), (WAVE_QUEUE, (64, 128, 256)),
-           (WAVE_QUEUE_GROUPS, (64, 128, 256)), (SCALAR_BOUNDS_PRIO, (128, 256)))
+           (WAVE_QUEUE_PRIO, (64, 128, 256)), (WAVE_QUEUE_GROUPS, (64, 128, 256)), (SCALAR_BOUNDS_PRIO, (128, 256)))
 
 
 def tile_unit(opts: Optional[str], block: int) -> int:

@@ -91,7 +91,7 @@ template <class S0, class S1> HD void batch_bounds_fill_impl(const BatchJob *__r
 // that need one); each pixel's samples run in order in one lane, as in
 // render_pixel_passes.
 template <class S>
-HD void batch_wave_tile(const BatchJob &job, uint first, uint end) {
+HD void batch_wave_tile(const BatchJob &job, uint first, uint end, bool tail) {
     const u8 *scene = reinterpret_cast<const u8 *>(job.scene);
     float4 *canvas = reinterpret_cast<float4 *>(job.canvas);
     uint *seeds = reinterpret_cast<uint *>(job.seeds);
@@ -100,7 +100,31 @@ HD void batch_wave_tile(const BatchJob &job, uint first, uint end) {
     const unsigned long long below = lane_mask_below();
     uint cursor = first;  // the same in every lane
     QueuedPixel<S> p;
+#ifdef HT_WAVE_PRIO
+    // On the frame's last tiles (`tail`): the wave's issue priority follows
+    // the work its tile has left (3 while pixels wait or its slowest lane has
+    // more than three quarters of the passes left, down to 0), so the waves
+    // that will finish last issue first. Back to 0 after the tile.
+    uint level = 0;
+#else
+    (void)tail;
+#endif
     for (;;) {
+#ifdef HT_WAVE_PRIO
+        if (tail) {
+            uint want = 0;
+            if (cursor < end || __ballot(p.have && p.done < passes / 4) != 0) want = 3;
+            else if (__ballot(p.have && p.done < passes / 2) != 0) want = 2;
+            else if (__ballot(p.have && p.done < passes - passes / 4) != 0) want = 1;
+            if (want != level) {
+                level = want;
+                if (want == 3) __builtin_amdgcn_s_setprio(3);
+                else if (want == 2) __builtin_amdgcn_s_setprio(2);
+                else if (want == 1) __builtin_amdgcn_s_setprio(1);
+                else __builtin_amdgcn_s_setprio(0);
+            }
+        }
+#endif
         if (cursor < end) {
             bool need = !p.have;
             unsigned long long m = __ballot(need);
@@ -114,6 +138,9 @@ HD void batch_wave_tile(const BatchJob &job, uint first, uint end) {
         }
         if (p.have) p.step(w, h, scene, canvas, seeds, passes, hops);
     }
+#ifdef HT_WAVE_PRIO
+    if (level != 0) __builtin_amdgcn_s_setprio(0);
+#endif
 }
 
 template <class S0, class S1>
@@ -131,10 +158,12 @@ HD void render_batch_impl(const BatchJob *__restrict__ jobs, uint njobs, uint *_
         uint npx = job.w * job.h;
         uint first = (t - job.tile_begin) * tile_px;
         uint end = first + (npx - first < tile_px ? npx - first : tile_px);
+        // one of the last tiles: fewer tiles remain than waves run
+        bool tail = t + gridDim.x * (blockDim.x / 64) >= total_tiles;
         if (job.type == 0) {
-            batch_wave_tile<S0>(job, first, end);
+            batch_wave_tile<S0>(job, first, end, tail);
         } else {
-            batch_wave_tile<S1>(job, first, end);
+            batch_wave_tile<S1>(job, first, end, tail);
         }
     }
 }

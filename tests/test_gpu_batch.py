@@ -43,7 +43,8 @@ def alone(ctx, spec, frames=1):
 IDENTITY_CASES = [(256, 1, None), (64, 4, None), (1024, 1, None), (256, 1, SCALAR_BOUNDS), (1024, 2, SCALAR_BOUNDS),
                   (128, 1, WAVE_QUEUE), (256, 4, WAVE_QUEUE), (64, 1, WAVE_QUEUE), (256, 0.25, WAVE_QUEUE),
                   (256, 1, SCALAR_BOUNDS + GROUPS4), (128, 0.5, WAVE_QUEUE + GROUPS4),
-                  (64, 2, WAVE_QUEUE + " -DHT_CULL_GROUPS=8"), (128, 1, SCALAR_BOUNDS + " -DHT_WAVE_PRIO=1")]
+                  (64, 2, WAVE_QUEUE + " -DHT_CULL_GROUPS=8"), (128, 1, SCALAR_BOUNDS + " -DHT_WAVE_PRIO=1"),
+                  (128, 1, WAVE_QUEUE + " -DHT_WAVE_PRIO=1"), (64, 2, WAVE_QUEUE + GROUPS4 + " -DHT_WAVE_PRIO=1")]
 GRAPH_OPTS = [None, SCALAR_BOUNDS, WAVE_QUEUE, WAVE_QUEUE + GROUPS4]  # 256-thread programs
 
 
