@@ -24,7 +24,11 @@ region. Beside it:
 * cpu_baseline: the CPU oracle (a restatement of the reference kernel) on
   rows of the same frame, on the host's cores.
 
-Prints one JSON line.
+The whole result (tuner tables, RMSE curves, every leg) goes to a JSON
+file (--detail, default gpurun_out/bench_detail.json); the last line of
+stdout is a summary of at most 2 KB: the headline, its roofline, the CPU
+baseline, the time-to-RMSE figures and one number per leg. Progress lines go
+to stderr.
 """
 from __future__ import annotations
 
@@ -69,13 +73,24 @@ def count_flops_per_path(scene_spec, w, h, seeds, stride=16):
     return {"fp32": culled32 / paths, "fp64": culled64 / paths, "plain_fp32": plain32 / paths, "paths": paths}
 
 
+def host_threads():
+    """Host CPUs this process may run on (its affinity mask), with what the
+    machine reports beside it."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return usable, {"os_cpu_count": os.cpu_count(), "affinity": usable,
+                    "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(scene_spec, w, h, seeds, budget_s=12.0):
     """Oracle FP32 on bands of rows of the frame, one sample per pixel per
     pass, pass after pass (seeds advancing as on the GPU), until about
-    `budget_s` of wall time."""
+    `budget_s` of wall time, on every host CPU the process may use."""
     import oracle
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads, host = host_threads()
     rows = 16
     canvas = np.zeros((h, w, 4), np.float32)
     seeds = np.array(seeds, dtype=np.uint32).reshape(h, w)
@@ -89,7 +104,109 @@ def cpu_baseline(scene_spec, w, h, seeds, budget_s=12.0):
     dt = time.perf_counter() - t0
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
             "sample": f"CPU oracle (FP32, OpenMP, {threads} threads) on {paths} paths of the same frame: "
-                      f"{bands} bands of {rows} rows x 1 spp, cycling over the frame, {dt:.1f} s"}
+                      f"{bands} bands of {rows} rows x 1 spp, cycling over the frame, {dt:.1f} s",
+            "host": host}
+
+
+SUMMARY_MAX_BYTES = 2048
+DETAIL_PATH = os.path.join(ROOT, "gpurun_out", "bench_detail.json")
+
+
+def _sig(x, digits=5):
+    """Floats rounded to `digits` significant digits (recursively)."""
+    if isinstance(x, float):
+        return float(f"{x:.{digits}g}")
+    if isinstance(x, dict):
+        return {k: _sig(v, digits) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_sig(v, digits) for v in x]
+    return x
+
+
+def _pick(d, keys):
+    d = d or {}
+    return {k: d[k] for k in keys if k in d}
+
+
+def summary(out, detail_path=None):
+    """The one-line result: the headline metric, its roofline, the CPU
+    baseline, the time-to-RMSE figures and one headline number per leg. The
+    tuner tables and RMSE curves stay in the detail file."""
+    s = _pick(out, ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
+                    "scaling", "vs_baseline", "dtype", "data"))
+    s["config"] = _pick(out.get("config"), ("workload", "width", "height", "spp", "hops", "primitives", "kernel"))
+    s["roofline"] = _pick(out.get("roofline"), ("bound", "achieved", "peak", "unit", "frac", "frac_nonpacked",
+                                                "kernel", "kernel_ms", "flops_per_path", "traffic",
+                                                "algorithmic_bytes"))
+    if "cpu_baseline" in out:
+        cb = out["cpu_baseline"]
+        s["cpu_baseline"] = _pick(cb, ("value", "unit", "cores", "kind", "sample"))
+        if "sample" in s["cpu_baseline"]:
+            s["cpu_baseline"]["sample"] = s["cpu_baseline"]["sample"][:200]
+    if "time_to_rmse" in out:
+        s["time_to_rmse"] = _pick(out["time_to_rmse"], ("target", "spp", "chosen_s", "plain_s", "speedup"))
+    legs = {}
+    for name in ("plain", "dropin_per_pass", "fast_fp"):
+        if name in out:
+            legs[name] = out[name].get("value")
+    eu = out.get("euclidean_320x240")
+    if eu:
+        legs["euclidean_320x240"] = ({"chosen": eu["chosen"]["value"], "plain": eu["plain"]["value"]}
+                                     if "chosen" in eu else {"error": str(eu.get("error"))[:120]})
+    rc = out.get("radiance_cache_1080p")
+    if rc:
+        legs["radiance_cache_1080p"] = (
+            {"bit_exact": _pick(rc.get("bit_exact_forms"), ("spp", "chosen_s", "plain_s", "speedup")),
+             "cached": _pick(rc.get("cached"), ("reached", "spp", "seconds", "speedup_vs_chosen"))}
+            if "cached" in rc else {"error": str(rc.get("error"))[:120]})
+    bt = out.get("batch_32_jobs")
+    if bt:
+        legs["batch_32_jobs"] = (
+            {"graph": bt["graph_replay"]["value"], "eager": bt["eager"]["value"],
+             "one_job_per_launch": bt["one_job_per_launch"]["value"],
+             "bit_identical": bt.get("bit_identical_to_single_jobs")}
+            if "eager" in bt else {"error": str(bt.get("error"))[:120]})
+    wf = out.get("wavefront_cells")
+    if wf:
+        legs["wavefront_cells"] = ({"ratio_to_megakernel": wf["wavefront_over_megakernel"],
+                                    "bit_identical": wf["bit_identical"]}
+                                   if "wavefront" in wf else {"error": str(wf.get("error"))[:120]})
+    sd = out.get("sdf_1080p")
+    if sd:
+        legs["sdf_1080p"] = sd["value"] if "value" in sd else {"error": str(sd.get("error"))[:120]}
+    s["legs"] = legs
+    if detail_path:
+        s["detail"] = os.path.relpath(detail_path, ROOT)
+    s = _sig(s)
+    # the line must stay small: drop legs, then the CPU sample text, if ever needed
+    for key in ("legs", "cpu_baseline"):
+        if len(json.dumps(s)) <= SUMMARY_MAX_BYTES:
+            break
+        if key == "legs":
+            s.pop("legs", None)
+        elif "cpu_baseline" in s:
+            s["cpu_baseline"].pop("sample", None)
+    return s
+
+
+def emit(out, detail_path=DETAIL_PATH, stream=None):
+    """Writes the whole result to `detail_path` and prints the summary as
+    the last line of stdout."""
+    stream = stream or sys.stdout
+    if detail_path:
+        try:
+            os.makedirs(os.path.dirname(detail_path), exist_ok=True)
+            with open(detail_path, "w") as f:
+                json.dump(out, f, indent=1)
+        except OSError as e:
+            print(f"bench: detail file not written: {e}", file=sys.stderr)
+            detail_path = None
+    stream.write(json.dumps(summary(out, detail_path)) + "\n")
+    stream.flush()
+
+
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def gamma_image(canvas):
@@ -518,6 +635,21 @@ def batch_leg(args):
                 b.render(solo["grid"], clear=True)
 
         single_s = timed(one_by_one, 2)
+        # one frame of the tuned batch program and of every job alone, both
+        # from fresh seeds: canvases and seeds must agree bit for bit
+        for b in singles:
+            b.close()
+        fresh = Batch(ctx, bprog, specs, best["tile"])
+        fresh.render(grid, clear=True)
+        same = True
+        for s, cb in zip(specs, fresh.canvases):
+            one = Batch(ctx, sprog, [s], solo["tile"])
+            one.render(solo["grid"], clear=True)
+            c1 = one.canvases[0]
+            same = same and bool(np.array_equal(cb.load_image().view(np.uint32), c1.load_image().view(np.uint32))
+                                 and np.array_equal(cb.load_seeds(), c1.load_seeds()))
+            one.close()
+        fresh.close()
     return {"workload": "32 jobs, half Euclidean / half hyperbolic, 128x128..1024x768, spp 1..64, depth 4..12, "
                         "8..256 primitives, diffuse/mirror/glass 0.6/0.25/0.15",
             "paths_per_frame": paths, "autotuned": best,
@@ -525,6 +657,7 @@ def batch_leg(args):
             "eager": {"ms_per_frame": eager_s * 1e3, "value": paths / eager_s / 1e6, "unit": "Mpaths/s"},
             "one_job_per_launch": {"ms_per_frame": single_s * 1e3, "value": paths / single_s / 1e6,
                                    "unit": "Mpaths/s", "program": solo},
+            "bit_identical_to_single_jobs": same,
             "tuner_table": tune["table"]}
 
 
@@ -554,6 +687,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "
                          "the known-size copy that scales FETCH_SIZE / WRITE_SIZE counter readings")
+    ap.add_argument("--detail", default=DETAIL_PATH,
+                    help="file for the whole result (tuner tables, RMSE curves); the last stdout line is a summary")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the render pass would shard by "
@@ -695,34 +830,27 @@ def main():
         "roofline": roofline,
     }
     out.update(extra)
-    if not args.no_euclidean and not args.no_rmse:
+    progress(f"headline {mpaths:.1f} Mpaths/s, {ms_per_step:.2f} ms per frame, roofline frac {roofline['frac']:.4f}")
+    # The legs after the headline; a failing leg is reported, the headline stays valid.
+    legs = [("euclidean_320x240", not args.no_euclidean and not args.no_rmse, lambda: euclidean_leg(args)),
+            ("radiance_cache_1080p", not args.no_cache and not args.no_rmse, lambda: cache_leg(args)),
+            ("batch_32_jobs", not args.no_batch, lambda: batch_leg(args)),
+            ("wavefront_cells", not args.no_wavefront,
+             lambda: wavefront_leg(args, out["render_tuning"]["best"]["options"])),
+            ("sdf_1080p", not args.no_sdf, lambda: sdf_leg(args))]
+    for name, wanted, run in legs:
+        if not wanted:
+            continue
+        t0 = time.perf_counter()
         try:
-            out["euclidean_320x240"] = euclidean_leg(args)
+            out[name] = run()
         except Exception as e:
-            out["euclidean_320x240"] = {"error": f"{type(e).__name__}: {e}"}
-    if not args.no_cache and not args.no_rmse:
-        try:
-            out["radiance_cache_1080p"] = cache_leg(args)
-        except Exception as e:  # the headline stays valid if this leg fails
-            out["radiance_cache_1080p"] = {"error": f"{type(e).__name__}: {e}"}
-    if not args.no_batch:
-        try:
-            out["batch_32_jobs"] = batch_leg(args)
-        except Exception as e:
-            out["batch_32_jobs"] = {"error": f"{type(e).__name__}: {e}"}
-    if not args.no_wavefront:
-        try:
-            out["wavefront_cells"] = wavefront_leg(args, out["render_tuning"]["best"]["options"])
-        except Exception as e:
-            out["wavefront_cells"] = {"error": f"{type(e).__name__}: {e}"}
-    if not args.no_sdf:
-        try:
-            out["sdf_1080p"] = sdf_leg(args)
-        except Exception as e:
-            out["sdf_1080p"] = {"error": f"{type(e).__name__}: {e}"}
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+        progress(f"{name} done in {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, W, H, seeds0)
-    print(json.dumps(out))
+        progress(f"cpu baseline {out['cpu_baseline']['value']:.3f} Mpaths/s on {out['cpu_baseline']['cores']} threads")
+    emit(out, args.detail)
 
 
 if __name__ == "__main__":

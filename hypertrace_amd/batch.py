@@ -20,7 +20,7 @@ from ctypes import Structure, c_uint32, c_void_p
 from typing import Dict, List, Optional, Sequence, Tuple
 
 from . import codegen, native
-from .processing import Buffer, Canvas, Context, EntityBuffer, ImageBuffer, Program, precompile
+from .processing import Buffer, Canvas, Context, EntityBuffer, ImageBuffer, Program, precompile, scene_type_key
 from .scenes import BatchJobSpec, batch_euclidean_scene, batch_hyperbolic_scene
 from .types import Config
 
@@ -68,6 +68,15 @@ class Batch:
         self.bprog = bprog
         self.tile = tile if tile is not None else bprog.block
         self.specs = list(specs)
+        # each job's scene must be of its type's scene type (Scene0 / Scene1
+        # of the batch program), or the kernel would read it with another layout
+        types = [scene_type_key(p, ctx.config) for p in prototype_scenes()]
+        for k, s in enumerate(self.specs):
+            if s.type not in (0, 1):
+                raise native.HtError(-3, f"job {k}: type {s.type} is neither 0 nor 1")
+            got = scene_type_key(s.scene, ctx.config)
+            if got != types[s.type]:
+                raise native.HtError(-3, f"job {k}: scene of type {got} is not the batch program's type {s.type}")
         self.scene_bufs = [EntityBuffer(ctx, s.scene) for s in self.specs]
         self.canvases = [Canvas(ctx, (s.width, s.height), 1, s.seed) for s in self.specs]
         self.rgba8 = [ImageBuffer(ctx, (s.width, s.height), 4, np.uint8) if with_rgba8 else None for s in self.specs]

@@ -164,6 +164,10 @@ def _suite_jobs():
             for spec in bt.edge_specs(prims):  # the single-job programs the jobs are compared with
                 sc = spec.scene
                 jobs.append((SceneImpl(sc.geo, sc.view, sc.object, sc.background, spec.depth), X32_F32, None, None))
+    ch = load("test_gpu_cache")
+    if ch is not None:
+        cells_type = scenes.hyperbolic_cells(64, 48).scene  # the program depends on the scene type only
+        jobs += [(cells_type, X32_F32, opts, None) for opts in getattr(ch, "CACHE_PROGRAMS", [])]
     wf = load("test_gpu_wavefront")
     if wf is not None:
         for make, opts in wf.CASES.values():
@@ -221,14 +225,36 @@ def precompile_suite_programs(verbose: bool = False, workers: int = 8) -> None:
         print(f"  test and bench programs: {n} ({compiled} compiled) in {time.time() - t0:.1f}s")
 
 
+CACHE_DIR = os.path.join(PKG, "_cache")
+
+
+def prune_cache(used_since: float, verbose: bool = False) -> int:
+    """Removes code objects no build step used since `used_since` (a cache
+    hit refreshes the entry's modification time): programs of earlier
+    device sources, which would only add to what travels with the tree."""
+    removed = 0
+    if not os.path.isdir(CACHE_DIR):
+        return 0
+    for name in os.listdir(CACHE_DIR):
+        path = os.path.join(CACHE_DIR, name)
+        if name.endswith(".hsaco") and os.path.getmtime(path) < used_since:
+            os.remove(path)
+            removed += 1
+    if verbose:
+        print(f"  removed {removed} unused code objects from the cache")
+    return removed
+
+
 def build_all(verbose: bool = True) -> None:
     t = time.time()
     build_library()
     if verbose:
         print(f"built {LIB} in {time.time() - t:.1f}s")
+    start = time.time() - 1.0
     precompile_programs(verbose)
     precompile_batch_programs(verbose)
     precompile_suite_programs(verbose)
+    prune_cache(start, verbose)
 
 
 if __name__ == "__main__":

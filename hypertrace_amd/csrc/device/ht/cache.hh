@@ -183,6 +183,10 @@ HD void render_cached_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
     color3 thr_q = c3(0.0f), em_q = c3(0.0f);
     const u8 *obj = S::object(scene);
     for (int i = 0; i < S::HOPS; ++i) {
+#ifdef HT_RR_DEPTH
+        // the roulette of bounce() (render.hh), at the same point of the hop
+        if (i >= HT_RR_DEPTH && russian_roulette(&ctx, &light.base)) break;
+#endif
         ctx.hasher = hash_init();
         typename T::Cache cache;
         real dist = TopScan<T>::detect(obj, &ctx, &cache, &light);

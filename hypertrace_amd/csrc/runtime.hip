@@ -8,6 +8,7 @@
 #include <dlfcn.h>
 #include <sys/stat.h>
 #include <unistd.h>
+#include <utime.h>
 
 #include <atomic>
 #include <cerrno>
@@ -393,6 +394,9 @@ int get_code_object(const char *root_name, const char *root_src, int n_headers, 
 
     if (read_file(path, code)) {
         g_cache_hit = 1;
+        // mark the entry as used (a build step prunes entries it did not use;
+        // a read-only cache directory is fine)
+        (void)utime(path.c_str(), nullptr);
         return HT_OK;
     }
     if (const char *log_compiles = std::getenv("HT_LOG_COMPILES")) {

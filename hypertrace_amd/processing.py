@@ -102,6 +102,25 @@ class EntityBuffer(Buffer):
         super().__init__(ctx, max(len(data), 1))
         self.write(data)
         self.scene_bytes = data
+        # The reference's EntityBuffer<S> fixes the scene type at compile
+        # time; here the type travels with the bytes and is checked before a
+        # program reads them (a buffer of another type would be read with
+        # the wrong layout, and its lengths could send reads past its end).
+        self.scene_type = scene_type_key(scene, ctx.config)
+
+
+def scene_type_key(scene: SceneImpl, config: Config) -> str:
+    """The scene type (its device type expression) and the layout config:
+    what a program and a scene buffer must agree on."""
+    return f"{scene.cpp()}|x{config.address_width}|f{64 if config.double_support else 32}"
+
+
+def check_scene_type(expected: Optional[str], buffer: "EntityBuffer") -> None:
+    """HtError (Error::Other) if `buffer` holds a scene of another type than
+    the program was built for."""
+    got = getattr(buffer, "scene_type", None)
+    if expected is not None and got != expected:
+        raise native.HtError(-3, f"scene buffer of type {got} given to a program built for {expected}")
 
 
 class ImageBuffer(Buffer):
@@ -198,6 +217,7 @@ class Program:
                                           fp_options.encode() if fp_options is not None else None, ctypes.byref(h)))
         self.cache_hit = bool(lib.ht_prog_cache_hit())
         self.handle = h
+        self.scene_type = scene_type_key(scene, ctx.config) if scene is not None else None
 
     def close(self) -> None:
         if self.handle:
@@ -271,21 +291,29 @@ class Render:
         self.program = Program(ctx, scene, fp_options=fp_options, source=source)
 
     def render(self, scene_buffer: EntityBuffer, canvas: Canvas) -> None:
+        check_scene_type(self.program.scene_type, scene_buffer)
         native.check(native.lib().ht_render(self.program.handle, canvas.width, canvas.height, scene_buffer.handle,
                                             canvas.image.handle, canvas.seeds.handle))
         canvas.add_pass()
 
     def render_passes(self, scene_buffer: EntityBuffer, canvas: Canvas, passes: int) -> None:
+        check_scene_type(self.program.scene_type, scene_buffer)
         native.check(native.lib().ht_render_passes(self.program.handle, canvas.width, canvas.height,
                                                    scene_buffer.handle, canvas.image.handle, canvas.seeds.handle,
                                                    passes))
         canvas.add_pass(passes)
 
     def render_wavefront(self, scene_buffer: EntityBuffer, canvas: Canvas, passes: int) -> None:
+        check_scene_type(self.program.scene_type, scene_buffer)
         native.check(native.lib().ht_render_wavefront(self.program.handle, canvas.width, canvas.height,
                                                       scene_buffer.handle, canvas.image.handle,
                                                       canvas.seeds.handle, passes))
         canvas.add_pass(passes)
+
+
+class _TypeOnly:
+    def __init__(self, key: str):
+        self.scene_type = key
 
 
 class Extractor:
@@ -346,6 +374,9 @@ class Pipeline:
         self.image_buf_packed = ImageBuffer(ctx, size, 4, np.uint8)
 
     def set_scene(self, scene: SceneImpl) -> None:
+        """Re-serialises and uploads the scene (pipeline.rs:44-47); the scene
+        must be of the type the pipeline's program was built for."""
+        check_scene_type(self.render_.program.scene_type, _TypeOnly(scene_type_key(scene, self.ctx.config)))
         self.scene_buffer = EntityBuffer(self.ctx, scene)
 
     def render(self) -> None:

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import native
 from .objects import SceneImpl
-from .processing import Buffer, Canvas, Context, EntityBuffer, Program
+from .processing import Buffer, Canvas, Context, EntityBuffer, Program, check_scene_type
 
 N_PARAMS = 26115
 N_FEATURES = 12
@@ -152,6 +152,7 @@ class CachedRender:
         self.passes_done = 0
 
     def render_pass(self, scene_buffer: EntityBuffer, canvas: Canvas, train: bool = True) -> None:
+        check_scene_type(self.program.scene_type, scene_buffer)
         lib = native.lib()
         native.check(lib.ht_render_cached(self.program.handle, self.cache.handle, canvas.width, canvas.height,
                                           scene_buffer.handle, canvas.image.handle, canvas.seeds.handle,

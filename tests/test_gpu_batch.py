@@ -8,7 +8,8 @@ import pytest
 from hypertrace_amd import scenes
 import dataclasses
 
-from hypertrace_amd.batch import SCALAR_BOUNDS, WAVE_QUEUE, Batch, BatchProgram
+from hypertrace_amd.batch import (SCALAR_BOUNDS, SCALAR_BOUNDS_PRIO, WAVE_QUEUE, WAVE_QUEUE_PRIO, Batch, BatchProgram,
+                                  autotune)
 from hypertrace_amd.native import HtError
 from hypertrace_amd.objects import SceneImpl
 from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
@@ -157,3 +158,60 @@ def test_batch_argument_checks(ctx):
 
     with pytest.raises(HtError, match="not a batch program"):
         Batch(ctx, NotBatch(), specs)
+
+
+# ---- config 3 at its workload: 32 jobs, half Euclidean / half hyperbolic,
+# 128x128..1024x768, spp 1..64, depth 4..12, 8..256 primitives
+# The candidates the test's autotune picks from (the bench tunes over all).
+CFG3_OPTIONS = ((WAVE_QUEUE_PRIO, (128,)), (SCALAR_BOUNDS_PRIO, (256,)))
+
+
+def test_config3_32_jobs_identical_to_single_renders(ctx):
+    """The 32-job frame of the batched workload in one persistent launch of
+    an autotuned batch program: every job's canvas and seeds equal
+    rendering that job alone with its own program."""
+    specs = scenes.batch_jobs()
+    assert len(specs) == 32
+    tune = autotune(ctx, specs, blocks=(128, 256), tile_mults=(1,), grid_waves_per_cu=(8,), reps=1,
+                    options=CFG3_OPTIONS)
+    best = tune["best"]
+    print(f"case config 3: {len(specs)} jobs, {tune['paths']} paths, best {best}", flush=True)
+    bprog = BatchProgram(ctx, best["block"], fp_options=best["options"])
+    batch = Batch(ctx, bprog, specs, best["tile"])
+    batch.render(best["grid"], clear=True)
+    ctx.sync()
+    for k, spec in enumerate(specs):
+        img, seeds = alone(ctx, spec)
+        got = batch.canvases[k]
+        np.testing.assert_array_equal(got.load_image().view(np.uint32), img.view(np.uint32), err_msg=f"job {k}")
+        np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
+        assert np.all(img[..., 3] == spec.spp)
+
+
+@pytest.mark.parametrize("geo_type", [0, 1])
+def test_config3_one_pass_crop_vs_oracle(ctx, geo_type):
+    """One pass of the first Euclidean (0) or hyperbolic (1) job of the
+    32-job set, rendered by a batch program, against the CPU oracle on a
+    crop: 1e-4 relative on 99.9 % of pixels, seeds equal."""
+    import oracle
+
+    spec = next(s for s in scenes.batch_jobs() if s.type == geo_type)
+    one = dataclasses.replace(spec, spp=1)
+    print(f"case config 3 crop: type {geo_type}, {one.width}x{one.height}, depth {one.depth}", flush=True)
+    bprog = BatchProgram(ctx, 128, fp_options=WAVE_QUEUE_PRIO)
+    batch = Batch(ctx, bprog, [one], 128)
+    seeds0 = batch.canvases[0].load_seeds().copy()
+    batch.render(grid=64, clear=True)
+    ctx.sync()
+    got, got_seeds = batch.canvases[0].load_image(), batch.canvases[0].load_seeds()
+    s = one.scene
+    scene = SceneImpl(s.geo, s.view, s.object, s.background, one.depth)
+    cw, ch = min(64, one.width), min(48, one.height)
+    x0, y0 = (one.width - cw) // 2, (one.height - ch) // 2
+    ref, ref_seeds, _ = oracle.render(scene.spec_text(X32_F32), one.width, one.height, 1, seeds0,
+                                      crop=(x0, y0, cw, ch))
+    sl = (slice(y0, y0 + ch), slice(x0, x0 + cw))
+    ok = np.all(np.abs(got[sl] - ref[sl]) <= 1e-4 * np.abs(ref[sl]) + 1e-6, axis=-1)
+    print(f"pixels within 1e-4: {ok.mean():.5f}, seeds equal {np.mean(got_seeds[sl] == ref_seeds[sl]):.5f}")
+    assert ok.mean() >= 0.999
+    assert np.mean(got_seeds[sl] == ref_seeds[sl]) >= 0.999

@@ -204,3 +204,127 @@ def test_ablation_trained_cache(ctx):
           f"cache trained {rmse_trained:.4f}")
     assert rmse_trained < 0.8 * rmse_untrained
     assert rmse_trained < 1.02 * rmse_plain
+
+
+# ---- config 5 at its workload: the cell scene at 1920x1080 with 256 primitives
+CFG5 = dict(width=1920, height=1080, primitives=256)
+# program options of the cached kernel the tests build beside the default
+# (precompiled by the build step)
+CACHE_PROGRAMS = [program_options(rr_depth=3)]
+
+
+@pytest.fixture(scope="module")
+def cfg5():
+    return scenes.hyperbolic_cells(CFG5["width"], CFG5["height"], 1, primitives=CFG5["primitives"])
+
+
+def _zero_cache(ctx, n, train_capacity=65536):
+    cache = RadianceCache(ctx, n, train_capacity, 16384)
+    cache.set_params(np.zeros(cache.params().size, np.float32))  # predicts 0 everywhere
+    return cache
+
+
+def test_config5_cache_off_equals_render(ctx, cfg5):
+    """query_hop >= HOPS with every pixel a training pixel: no path ends into
+    the cache, so one cached pass is one pass of the reference loop: canvas
+    and seeds bit for bit those of `render`, over the full 1920x1080 frame."""
+    wl = cfg5
+    W, H = wl.width, wl.height
+    print(f"case config 5 cache off {W}x{H}", flush=True)
+    buf = EntityBuffer(ctx, wl.scene)
+    plain = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    Render(ctx, wl.scene).render(buf, plain)
+    cache = _zero_cache(ctx, W * H)
+    cr = CachedRender(ctx, wl.scene, cache, query_hop=wl.scene.hops, train_every=1, steps_per_pass=0)
+    c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    cr.render_pass(buf, c, train=False)
+    ctx.sync()
+    assert cache.counts() == (0, 0)
+    np.testing.assert_array_equal(c.load_image().view(np.uint32), plain.load_image().view(np.uint32))
+    np.testing.assert_array_equal(c.load_seeds(), plain.load_seeds())
+
+
+def test_cache_off_equals_render_with_russian_roulette(ctx):
+    """The same equivalence for a program with Russian roulette: the cached
+    kernel's hop loop draws the roulette where bounce() does."""
+    wl = scenes.hyperbolic_cells(160, 120, 1)
+    opts = CACHE_PROGRAMS[0]
+    buf = EntityBuffer(ctx, wl.scene)
+    plain = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
+    r = Render(ctx, wl.scene, fp_options=opts)
+    cache = _zero_cache(ctx, wl.width * wl.height)
+    cr = CachedRender(ctx, wl.scene, cache, query_hop=wl.scene.hops, train_every=1, steps_per_pass=0,
+                      fp_options=opts)
+    c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
+    for _ in range(3):
+        r.render(buf, plain)
+        cr.render_pass(buf, c, train=False)
+    ctx.sync()
+    np.testing.assert_array_equal(c.load_image().view(np.uint32), plain.load_image().view(np.uint32))
+    np.testing.assert_array_equal(c.load_seeds(), plain.load_seeds())
+
+
+def test_config5_unqueried_pixels_match_render_and_oracle(ctx, cfg5):
+    """One cached pass (paths end into the cache at their third hit, one
+    pixel in 16 trains) with a network that predicts 0: every pixel whose
+    path did not end into the cache equals one `render` pass bit for bit
+    (canvas and seeds), the pixels whose seeds differ are exactly the queued
+    queries (each queried path skips the mixture draw of its third hit), and
+    on a crop the render pass matches the CPU oracle at the parity bar."""
+    import oracle
+
+    wl = cfg5
+    W, H = wl.width, wl.height
+    print(f"case config 5 unqueried pixels {W}x{H}", flush=True)
+    buf = EntityBuffer(ctx, wl.scene)
+    plain = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    seeds0 = plain.load_seeds().copy()
+    Render(ctx, wl.scene).render(buf, plain)
+    cache = _zero_cache(ctx, W * H)
+    cr = CachedRender(ctx, wl.scene, cache, query_hop=2, train_every=16, steps_per_pass=0)
+    c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    cr.render_pass(buf, c, train=False)
+    ctx.sync()
+    queries, records = cache.counts()
+    a, sa = plain.load_image(), plain.load_seeds()
+    b, sb = c.load_image(), c.load_seeds()
+    queried = sa != sb
+    print(f"queries {queries}, training records {records}, pixels with other seeds {int(queried.sum())}")
+    assert 0 < queries < W * H and 0 < records <= 65536
+    assert int(queried.sum()) == queries
+    np.testing.assert_array_equal(b[~queried].view(np.uint32), a[~queried].view(np.uint32))
+    assert np.all(np.isfinite(b)) and np.all(b[..., 3] == 1.0) and np.all(b[..., :3] >= 0.0)
+    # a 64x32 crop through the middle of the frame against the CPU oracle
+    x0, y0, cw, ch = 928, 524, 64, 32
+    ref, ref_seeds, _ = oracle.render(wl.scene.spec_text(X32_F32), W, H, 1, seeds0, crop=(x0, y0, cw, ch))
+    sl = (slice(y0, y0 + ch), slice(x0, x0 + cw))
+    ok = np.all(np.abs(a[sl] - ref[sl]) <= 1e-4 * np.abs(ref[sl]) + 1e-6, axis=-1)
+    print(f"crop vs oracle: {ok.mean():.5f} of pixels within 1e-4, seeds equal {np.mean(sa[sl] == ref_seeds[sl]):.5f}")
+    assert ok.mean() >= 0.999
+    keep = ~queried[sl]
+    assert keep.sum() > 0
+    ok_b = np.all(np.abs(b[sl][keep] - ref[sl][keep]) <= 1e-4 * np.abs(ref[sl][keep]) + 1e-6, axis=-1)
+    assert ok_b.mean() >= 0.999
+
+
+def test_config5_trained_frame_properties(ctx, cfg5):
+    """Four trained cached passes over the full frame (4 Adam steps of 16384
+    suffixes each): finite, non-negative radiance, alpha = passes, queue
+    counts within capacity, finite parameters that moved."""
+    wl = cfg5
+    W, H = wl.width, wl.height
+    print(f"case config 5 trained passes {W}x{H}", flush=True)
+    buf = EntityBuffer(ctx, wl.scene)
+    cache = RadianceCache(ctx, W * H, 65536, 16384, seed=3)
+    p0 = cache.params().copy()
+    cr = CachedRender(ctx, wl.scene, cache, query_hop=2, train_every=16, steps_per_pass=4, batch=16384)
+    c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    for _ in range(4):
+        cr.render_pass(buf, c)
+        q, r = cache.counts()
+        assert 0 < q <= W * H and 0 < r <= 65536
+    ctx.sync()
+    img = c.load_image()
+    p1 = cache.params()
+    assert np.all(np.isfinite(img)) and np.all(img[..., :3] >= 0.0) and np.all(img[..., 3] == 4.0)
+    assert np.all(np.isfinite(p1)) and not np.array_equal(p0, p1)

@@ -1,0 +1,80 @@
+"""The host pipeline's frame-loop semantics on the GPU, as the reference's
+examples drive them (main/examples/hy.rs:185-205): `clean()` zeroes the
+image and the pass count but keeps the seeds (processing/src/canvas.rs:102-106),
+and `set_scene` re-serialises and uploads the scene before a render
+(processing/src/pipeline.rs:44-47), so the next passes continue each
+pixel's RNG stream over the new scene."""
+import numpy as np
+import pytest
+
+from hypertrace_amd import scenes
+from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Pipeline, Render
+from hypertrace_amd.types import X32_F32
+
+pytestmark = pytest.mark.gpu
+
+W, H = 96, 72
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0, X32_F32)
+    yield c
+    c.close()
+
+
+def test_canvas_clean_keeps_seeds(ctx):
+    wl = scenes.hyperbolic_cells(W, H, 1)
+    render = Render(ctx, wl.scene)
+    buf = EntityBuffer(ctx, wl.scene)
+    c = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    seeds0 = c.load_seeds().copy()
+    render.render_passes(buf, c, 3)
+    seeds3 = c.load_seeds().copy()
+    assert not np.array_equal(seeds3, seeds0)
+    c.clean()
+    assert c.passes == 0
+    assert np.all(c.load_image() == 0.0)
+    np.testing.assert_array_equal(c.load_seeds(), seeds3)
+    # the next pass continues the RNG streams: the same as a fresh canvas
+    # holding those seeds
+    render.render(buf, c)
+    fresh = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+    fresh.store_seeds(seeds3)
+    render.render(buf, fresh)
+    assert c.passes == 1
+    np.testing.assert_array_equal(c.load_image().view(np.uint32), fresh.load_image().view(np.uint32))
+    np.testing.assert_array_equal(c.load_seeds(), fresh.load_seeds())
+
+
+def test_pipeline_set_scene_then_render(ctx):
+    """A frame loop: render a pass of scene A, move to scene B (same type,
+    other values) with clean() + set_scene(), render a pass: the canvas is
+    B's pass from the seeds A's pass left, and the RGBA8 frame is its post
+    pass."""
+    a = scenes.hyperbolic_cells(W, H, 1, primitives=192)
+    b = scenes.hyperbolic_cells(W, H, 1, primitives=150)
+    p = Pipeline(ctx, (W, H), a.scene, seed_mode=a.seed_mode, seed=a.seed)
+    p.render()
+    seeds_a = p.canvas.load_seeds().copy()
+    p.clean()
+    p.set_scene(b.scene)
+    p.render()
+    rgba = p.get_image()
+    # the same pass by hand
+    render_b = Render(ctx, b.scene)
+    buf_b = EntityBuffer(ctx, b.scene)
+    c = Canvas(ctx, (W, H), a.seed_mode, a.seed)
+    c.store_seeds(seeds_a)
+    render_b.render(buf_b, c)
+    np.testing.assert_array_equal(p.canvas.load_image().view(np.uint32), c.load_image().view(np.uint32))
+    np.testing.assert_array_equal(p.canvas.load_seeds(), c.load_seeds())
+    # and it differs from scene A's pass from the same seeds
+    render_a = Render(ctx, a.scene)
+    buf_a = EntityBuffer(ctx, a.scene)
+    ca = Canvas(ctx, (W, H), a.seed_mode, a.seed)
+    ca.store_seeds(seeds_a)
+    render_a.render(buf_a, ca)
+    assert not np.array_equal(ca.load_image(), c.load_image())
+    assert rgba.shape == (H, W, 4) and np.all(rgba[..., 3] == 255)
+    assert np.array_equal(p.get_image(fused=True), rgba)
