@@ -35,6 +35,7 @@ SIGNATURES = {
     "ht_render": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p]),
     "ht_render_passes": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32]),
     "ht_render_wavefront": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32]),
+    "ht_render_wavefront_noalloc": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_uint32]),
     "ht_extract": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p]),
     "ht_gamma": (c_int, [c_void_p, c_uint32, c_uint32, c_float, c_void_p, c_void_p]),
     "ht_pack": (c_int, [c_void_p, c_uint32, c_uint32, c_void_p, c_void_p]),

@@ -126,6 +126,21 @@ int ht_render_passes(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_bu
 int ht_render_wavefront(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas_f32x4,
                         ht_buf *seeds_u32, uint32_t passes);
 
+/* ht_render_wavefront without any allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * queues live in one fixed array of HT_WF_SCRATCH_BYTES per GPU, part of the
+ * library and held by each GPU it runs on until the process ends, instead
+ * of the program's workspace; frames whose queues need more (about 550 B
+ * per pixel for the cell scene: ~120 k pixels) are refused with
+ * HT_ERR_OTHER. Every call on a GPU shares that GPU's array, so calls are
+ * serialised within a process by a lock and each returns with its stream
+ * idle; calls that overlapped on one GPU would race. Same canvas and seeds as
+ * ht_render_wavefront. Synchronous. */
+#define HT_WF_SCRATCH_BYTES (64u << 20)
+int ht_render_wavefront_noalloc(ht_prog *prog, uint32_t w, uint32_t h, ht_buf *scene, ht_buf *canvas_f32x4,
+                                ht_buf *seeds_u32, uint32_t passes);
+
 /* Post passes (asynchronous until a read). image, out: float4[w*h];
  * rgba8: uchar4[w*h].
  * ht_extract: image = canvas / (float)passes   (processing/src/canvas.rs:126-135)
