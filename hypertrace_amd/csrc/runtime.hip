@@ -230,13 +230,25 @@ __global__ void __launch_bounds__(64) k_mo_compose(uint8_t *__restrict__ scene, 
 
 namespace {
 
-// SplitMix64 step as used by rand 0.8's Xoshiro256PlusPlus::seed_from_u64.
+// SplitMix64 step: the seeding of rand_xoshiro's own
+// Xoshiro256PlusPlus::seed_from_u64 (and of an all-zero from_seed).
 uint64_t splitmix64(uint64_t *state) {
     *state += 0x9e3779b97f4a7c15ull;
     uint64_t z = *state;
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     return z ^ (z >> 31);
+}
+
+// One word of rand_core 0.6's default SeedableRng::seed_from_u64: advance a
+// 64-bit LCG state (PCG's multiplier and a fixed increment), then the PCG32
+// XSH-RR output of the new state.
+uint32_t pcg32_seed_word(uint64_t *state) {
+    *state = *state * 6364136223846793005ull + 11634580027462260723ull;
+    uint64_t s = *state;
+    uint32_t xorshifted = (uint32_t)(((s >> 18) ^ s) >> 27);
+    uint32_t rot = (uint32_t)(s >> 59);
+    return (xorshifted >> rot) | (xorshifted << ((32u - rot) & 31u));
 }
 
 inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
@@ -271,11 +283,24 @@ extern "C" uint32_t ht_seed_hash(uint64_t seed, uint32_t index) {
 
 extern "C" int ht_seed_fill(uint32_t *dst, size_t n, int mode, uint64_t seed) {
     if (!dst && n) return fail(HT_ERR_OTHER, "ht_seed_fill: null destination");
-    if (mode == 0) {
+    if (mode == 0 || mode == 2) {
         Xoshiro256pp rng;
         uint64_t st = seed;
-        for (int i = 0; i < 4; ++i) rng.s[i] = splitmix64(&st);
+        if (mode == 0) {
+            // SmallRng::seed_from_u64 of rand 0.8: SmallRng does not override
+            // the trait's default, which fills the 32-byte seed with 8 words of
+            // pcg32_seed_word (little-endian); Xoshiro256PlusPlus::from_seed
+            // reads it as 4 little-endian u64.
+            for (int i = 0; i < 4; ++i) {
+                uint64_t lo = pcg32_seed_word(&st), hi = pcg32_seed_word(&st);
+                rng.s[i] = lo | (hi << 32);
+            }
+        } else {
+            // rand_xoshiro's Xoshiro256PlusPlus::seed_from_u64: SplitMix64
+            for (int i = 0; i < 4; ++i) rng.s[i] = splitmix64(&st);
+        }
         if (!(rng.s[0] | rng.s[1] | rng.s[2] | rng.s[3])) {
+            // from_seed of an all-zero seed: seed_from_u64(0) of the generator
             st = 0;
             for (int i = 0; i < 4; ++i) rng.s[i] = splitmix64(&st);
         }

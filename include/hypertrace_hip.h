@@ -263,9 +263,13 @@ int ht_batch_replay(ht_batch *batch);
 /* Per-pixel seed buffer. Replaces Canvas::new's seed fill
  * (processing/src/canvas.rs:73-84).
  * mode 0: rand 0.8 SmallRng::seed_from_u64(seed).fill(&mut [u32]) on 64-bit
- *         targets (xoshiro256++ seeded by SplitMix64); the reference uses
- *         seed = 0xdeadbeefdeadbeef.
- * mode 1: seeds[i] = ht_seed_hash(seed, i) (per-pixel hash seeding). */
+ *         targets: xoshiro256++ whose 32-byte seed is rand_core 0.6's
+ *         default seed_from_u64 expansion (8 PCG32 words), since SmallRng
+ *         does not override it; the reference uses seed = 0xdeadbeefdeadbeef.
+ * mode 1: seeds[i] = ht_seed_hash(seed, i) (per-pixel hash seeding).
+ * mode 2: as mode 0 with the generator seeded by SplitMix64, as
+ *         rand_xoshiro's own Xoshiro256PlusPlus::seed_from_u64 does (what a
+ *         SmallRng that forwarded seed_from_u64 to its generator would give). */
 int ht_seed_init(ht_buf *seeds, uint32_t w, uint32_t h, int mode, uint64_t seed);
 /* Host-side seed generation, same modes, into a caller array of n words. */
 int ht_seed_fill(uint32_t *dst, size_t n, int mode, uint64_t seed);

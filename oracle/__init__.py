@@ -51,6 +51,13 @@ def lib():
         l.oracle_sdf_detect.restype = c_double
         l.oracle_sdf_detect.argtypes = [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
         l.oracle_post.argtypes = [c_void_p, c_size_t, c_uint32, c_float, c_void_p, c_void_p]
+        l.oracle_material_interact.restype = c_int
+        l.oracle_material_interact.argtypes = [c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, POINTER(c_uint32),
+                                               c_void_p, c_void_p, c_void_p]
+        l.oracle_background.restype = c_int
+        l.oracle_background.argtypes = [c_char_p, c_void_p, c_void_p, c_void_p]
+        l.oracle_tiled_select.restype = c_int
+        l.oracle_tiled_select.argtypes = [c_int, c_int, c_double, c_double, c_int, c_void_p]
         _lib = l
     return _lib
 
@@ -123,6 +130,44 @@ def sdf_detect(kind: int, params, start, direction, repeat: bool = False):
     os_, on = np.zeros(3), np.zeros(3)
     dist = lib().oracle_sdf_detect(kind, _p(pr), int(repeat), _p(s), _p(d), _p(os_), _p(on))
     return float(dist), os_, on
+
+
+def material_interact(spec: str, normal, direction, intensity=(1.0, 1.0, 1.0), emission=(0.0, 0.0, 0.0),
+                      rng_state: int = 1):
+    """One material interaction (FP64 geometry, FP32 colours). Returns
+    (goes_on, direction, intensity, emission, rng_state after)."""
+    n = np.ascontiguousarray(normal, np.float64)
+    d = np.ascontiguousarray(direction, np.float64)
+    i = np.ascontiguousarray(intensity, np.float32)
+    e = np.ascontiguousarray(emission, np.float32)
+    st = c_uint32(rng_state)
+    od, oi, oe = np.zeros(3), np.zeros(3, np.float32), np.zeros(3, np.float32)
+    r = lib().oracle_material_interact(spec.encode(), _p(n), _p(d), _p(i), _p(e), ctypes.byref(st), _p(od), _p(oi),
+                                       _p(oe))
+    if r < 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return bool(r), od, oi, oe, int(st.value)
+
+
+def background(spec: str, direction, intensity=(1.0, 1.0, 1.0)):
+    """Emission of a Euclidean background for an escaping path."""
+    d = np.ascontiguousarray(direction, np.float64)
+    i = np.ascontiguousarray(intensity, np.float32)
+    out = np.zeros(3, np.float32)
+    if lib().oracle_background(spec.encode(), _p(d), _p(i), _p(out)) != 0:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return out
+
+
+def tiled_select(obj: int, tiling: int, cell: float, border: float, n_materials: int, p) -> int:
+    """Material index a tiled object picks at p (-1: border). obj 0 = tiled
+    plane, 1 = tiled horosphere."""
+    q = np.zeros(4, np.float64)
+    q[: len(p)] = p
+    r = lib().oracle_tiled_select(obj, tiling, cell, border, n_materials, _p(q))
+    if r < -1:
+        raise ValueError("bad tiled-object argument")
+    return r
 
 
 def post(canvas: np.ndarray, passes: int, gamma: float):

@@ -21,7 +21,12 @@
 //
 // Which check pins each part: tests/test_oracle_pins.py (leaf functions
 // against the reference's compiled leaf sources, closed forms and brute
-// force), tests/test_oracle_scenes.py (whole paths against closed forms).
+// force), tests/test_oracle_materials.py (materials, mixture selection,
+// backgrounds and the tiled objects against the reference's own unit-test
+// rules and the tilings' geometry, through the oracle_material_interact /
+// oracle_background / oracle_tiled_select entry points), tests/test_oracle_sdf.py
+// (sphere tracing, roulette), tests/test_oracle_scenes.py (whole paths
+// against closed forms). Every function here has such a check.
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -1784,6 +1789,97 @@ double oracle_sdf_detect(int kind, const double *params, int repeat, const doubl
     out_normal[1] = n.y;
     out_normal[2] = n.z;
     return d;
+}
+
+// One material interaction in FP64 geometry with FP32 colours (the
+// reference's DOUBLE_PRECISION build): `spec` is a material in the scene
+// spec's grammar ("specular", "mixture 2 0.3 absorbing 0.7 lambertian", ...);
+// the light arrives along `dir` with `intensity` at a surface of local
+// normal `normal`, emission starts at `emission_in`. Writes the outgoing
+// direction, intensity and emission, advances *rng_state by the draws the
+// material takes, and returns 1 if the path goes on, 0 if it ends, -1 on a
+// parse error.
+int oracle_material_interact(const char *spec, const double *normal, const double *dir, const float *intensity,
+                             const float *emission_in, uint32_t *rng_state, double *out_dir, float *out_intensity,
+                             float *out_emission) {
+    try {
+        Tokens tk(spec);
+        Parser<double, float, GeoEu<double>> p(tk, false);
+        auto m = p.material();
+        Ctx ctx{};
+        ctx.rng.state = *rng_state;
+        LightLocal<double, float> ll;
+        ll.intensity = {intensity[0], intensity[1], intensity[2]};
+        ll.dir = {dir[0], dir[1], dir[2]};
+        C3<float> e = {emission_in[0], emission_in[1], emission_in[2]};
+        bool ret = m->interact(ctx, V3<double>{normal[0], normal[1], normal[2]}, ll, e);
+        *rng_state = ctx.rng.state;
+        out_dir[0] = ll.dir.x;
+        out_dir[1] = ll.dir.y;
+        out_dir[2] = ll.dir.z;
+        out_intensity[0] = ll.intensity.x;
+        out_intensity[1] = ll.intensity.y;
+        out_intensity[2] = ll.intensity.z;
+        out_emission[0] = e.x;
+        out_emission[1] = e.y;
+        out_emission[2] = e.z;
+        return ret ? 1 : 0;
+    } catch (const std::exception &ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+// Emission a Euclidean background adds for a path escaping along `dir` with
+// `intensity` (FP64 geometry, FP32 colours); `spec` in the scene spec's
+// grammar ("gradbg dx dy dz r0 g0 b0 r1 g1 b1 power", "constbg r g b").
+int oracle_background(const char *spec, const double *dir, const float *intensity, float *out_emission) {
+    try {
+        Tokens tk(spec);
+        Parser<double, float, GeoEu<double>> p(tk, false);
+        auto bg = p.background();
+        Light<double, float, GeoEu<double>> l{};
+        l.ray.dir = {dir[0], dir[1], dir[2]};
+        l.intensity = {intensity[0], intensity[1], intensity[2]};
+        C3<float> e = {0.0f, 0.0f, 0.0f};
+        bg->interact(l, e);
+        out_emission[0] = e.x;
+        out_emission[1] = e.y;
+        out_emission[2] = e.z;
+        return 0;
+    } catch (const std::exception &ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+// Material a tiled object picks at the point p of its surface (FP64):
+// object 0 = tiled plane (tiling 1 pentagonal, 2 pentastar), 1 = tiled
+// horosphere (tiling 1 square, 2 hexagonal); returns the material's index
+// 0..n_materials-1, -1 for the border material, -2 on a bad argument.
+int oracle_tiled_select(int object, int tiling, double cell, double border, int n_materials, const double *p) {
+    if (n_materials < 1 || n_materials > 64) return -2;
+    auto fill = [&](auto *o) {
+        o->tiling = tiling;
+        o->cell = cell;
+        o->border = border;
+        for (int i = 0; i < n_materials; ++i) o->mats.emplace_back(new Transparent<double, float>());
+        o->border_mat.reset(new Transparent<double, float>());
+        const Material<double, float> *m = o->select(V4<double>{p[0], p[1], p[2], p[3]});
+        if (m == o->border_mat.get()) return -1;
+        for (int i = 0; i < n_materials; ++i)
+            if (m == o->mats[i].get()) return i;
+        return -2;
+    };
+    if (object == 0) {
+        TiledPlaneObj<double, float> o;
+        return fill(&o);
+    }
+    if (object == 1) {
+        TiledHoroObj<double, float> o;
+        return fill(&o);
+    }
+    return -2;
 }
 
 // Post passes on the CPU: extract -> gamma -> pack (processing/src/canvas.rs:126-135,

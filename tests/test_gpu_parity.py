@@ -160,7 +160,7 @@ def test_multi_pass_parity_hyperbolic(ctx):
 
 
 def test_seed_init_matches_host(ctx):
-    for mode, seed in ((0, 0xDEADBEEFDEADBEEF), (1, 0x48595054 ^ 2)):
+    for mode, seed in ((0, 0xDEADBEEFDEADBEEF), (1, 0x48595054 ^ 2), (2, 0xDEADBEEFDEADBEEF)):
         c = Canvas(ctx, (37, 11), mode, seed)
         np.testing.assert_array_equal(c.load_seeds().reshape(-1), seed_fill(37 * 11, mode, seed))
 
