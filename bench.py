@@ -54,17 +54,29 @@ NO_CULL = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_NO_CULL=1"
 ORACLE_SEED_XOR = 0x0BAD5EED
 
 
-def count_flops_per_path(scene_spec, w, h, seeds, stride=16):
+def count_mode(knobs):
+    """The oracle's counting mode for the scan a tuner setting runs: 1 = the
+    flat culled scan; g = two-level over groups of g (the group tests, then
+    the leaf tests of the groups each ray passes); 1000 + g = the same with
+    the group queue (the pair's cull state rebuilt per passing group)."""
+    g = int(knobs.get("groups") or 0)
+    if not g or not knobs.get("scalar_bounds"):
+        return 1  # groups over LDS bounds build their tree per block; counted as the flat scan's tests
+    return (1000 if knobs.get("group_queue") else 0) + g
+
+
+def count_flops_per_path(scene_spec, w, h, seeds, stride=16, mode=1):
     """Flops per path counted by the oracle's counting scalars on every pixel
-    of every stride-th row (first pass of each): the culled scan's algorithm
-    (FP32 exact tests of the candidates + FP64 cull tests) and the plain
-    reference algorithm (every element's exact test)."""
+    of every stride-th row (first pass of each): the algorithm the chosen
+    program's scan runs (`mode`, see count_mode: FP32 exact tests of the
+    candidates + the FP64 cull tests) and the plain reference algorithm
+    (every element's exact test)."""
     import oracle
 
     culled32 = culled64 = plain32 = 0
     paths = 0
     for y in range(0, h, stride):
-        _, _, c32, c64 = oracle.count_flops(scene_spec, w, h, 1, seeds, True, crop=(0, y, w, 1))
+        _, _, c32, c64 = oracle.count_flops(scene_spec, w, h, 1, seeds, mode, crop=(0, y, w, 1))
         _, _, p32, _ = oracle.count_flops(scene_spec, w, h, 1, seeds, False, crop=(0, y, w, 1))
         culled32 += c32
         culled64 += c64
@@ -467,19 +479,21 @@ def cache_leg(args):
 
 
 def fixed_knobs(text, base=None):
-    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p]" (no timing)."""
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq]" (no timing)."""
     from hypertrace_amd.processing import STRICT_FP
     from hypertrace_amd.tuning import options
     knobs = text.split(",")
     unroll, waves = int(knobs[0]), int(knobs[1])
-    sb = "sb" in knobs[2:]
-    groups = next((int(k[1:]) for k in knobs[2:] if k.startswith("g")), 0)
-    queue = "q" in knobs[2:]
-    prio = "p" in knobs[2:]
+    flags = knobs[2:]
+    sb = "sb" in flags
+    groups = next((int(k[1:]) for k in flags if k.startswith("g") and k[1:].isdigit()), 0)
+    queue = "q" in flags
+    prio = "p" in flags
+    gq = "gq" in flags
     return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                     "pixel_queue": queue, "wave_prio": prio,
+                     "pixel_queue": queue, "wave_prio": prio, "group_queue": gq,
                      "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups,
-                                        pixel_queue=queue, wave_prio=prio)},
+                                        pixel_queue=queue, wave_prio=prio, group_queue=gq)},
             "table": [], "fixed": True}
 
 
@@ -679,10 +693,10 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
-    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p]",
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq]",
                     help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
                          "instead of tuning (keeps a profiled trace to one program)")
-    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p]",
+    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq]",
                     help="the same for the SDF leg's program")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "
@@ -794,7 +808,7 @@ def main():
             chosen.render_passes(buf, frame, SPP)
             extra["fp64_error"] = same_seed_fp64_error(wl, SPP, frame.load_image())
 
-    fl = count_flops_per_path(spec, W, H, seeds0)
+    fl = count_flops_per_path(spec, W, H, seeds0, mode=count_mode(tuning["best"]))
     flops_per_path = fl["fp32"] + fl["fp64"]
     achieved = flops_per_path * paths_per_step / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK, "unit": "TFLOP/s",
@@ -803,8 +817,9 @@ def main():
                 "flops_per_path": flops_per_path, "fp32_flops_per_path": fl["fp32"],
                 "fp64_flops_per_path": fl["fp64"], "flops_sample_paths": fl["paths"],
                 "flops_note": "algorithmic flops of the algorithm the kernel runs (FP32 exact tests of the "
-                              "candidates, FP64 cull tests; FP64 and non-packed FP32 issue at the same rate), "
-                              "counted by the CPU oracle on the same paths",
+                              "candidates, FP64 cull tests of the chosen scan; FP64 and non-packed FP32 issue at the "
+                              "same rate), counted by the CPU oracle on the same paths",
+                "flops_count_mode": count_mode(tuning["best"]),
                 "plain_flops_per_path": fl["plain_fp32"],
                 "plain_equivalent_tflops": fl["plain_fp32"] * paths_per_step / (kernel_ms * 1e-3) / 1e12,
                 "traffic": MEASURED_TRAFFIC_BYTES,

@@ -63,6 +63,23 @@ static constexpr int WAVE = 64;
 static constexpr uint RING_STEP = HT_RING_STEP;
 static constexpr uint QUEUE_RING = WAVE * (RING_STEP + 1);
 HD uint ring_wrap(uint i) { return i >= QUEUE_RING ? i - QUEUE_RING : i; }
+// Program option HT_GROUP_QUEUE (with HT_SCALAR_BOUNDS and HT_CULL_GROUPS):
+// the two-level scan compacts at both levels. The group spheres are tested
+// by every lane (wave-uniform scalar loads); the passing (group, ray) pairs
+// go to a second per-wave ring, drained in batches of one pair per active
+// lane: each lane tests the pair's group leaves (staged in LDS per block)
+// for the pair's ray and pushes the passing (element, ray) pairs to the
+// exact tests' ring. A ray then pays for the groups it passes, not for the
+// groups any ray of its wave passes. The group ring holds the pairs left
+// after a batch (< 64) plus one level-1 step's appends (<= 64 per group).
+static constexpr uint GROUP_RING = WAVE * (HT_CULL_UNROLL + 1);
+HD uint gring_wrap(uint i) { return i >= GROUP_RING ? i - GROUP_RING : i; }
+#if defined(HT_GROUP_QUEUE) && !(defined(HT_SCALAR_BOUNDS) && defined(HT_CULL_GROUPS))
+#error "HT_GROUP_QUEUE needs HT_SCALAR_BOUNDS and HT_CULL_GROUPS"
+#endif
+#if defined(HT_GROUP_QUEUE) && defined(HT_BATCH_BLOCK)
+#error "HT_GROUP_QUEUE is an option of scene programs"
+#endif
 #ifndef HT_MAX_WAVES
 #define HT_MAX_WAVES 4  // blocks of up to 256 threads
 #endif
@@ -438,6 +455,9 @@ template <class G> struct WaveScratch {
     uint prev_hash[WAVE];
     unsigned long long best[WAVE];
     uint queue[QUEUE_RING];
+#ifdef HT_GROUP_QUEUE
+    uint gqueue[GROUP_RING];  // (group << 6) | ray lane
+#endif
 };
 
 template <class Tag> HD Bound *bounds_lds() {
@@ -590,6 +610,7 @@ template <class T> struct TopScan {
     }
 };
 
+
 template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     typedef ObjectVec<G, E> OV;
     typedef typename OV::V V;
@@ -604,7 +625,32 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     // All threads of the block, before any of them diverges; the caller
     // synchronises the block afterwards.
     static HD void prepare(const u8 *self) {
-#ifdef HT_SCALAR_BOUNDS
+#if defined(HT_GROUP_QUEUE)
+        // the tree's leaves (Morton order, element index in pad1) from the
+        // global array bounds_fill wrote, narrowed to FP32: their values are
+        // FP32 numbers widened (the squares FP64-exact), so the narrowed
+        // bound decides exactly as the FP64 one
+        if constexpr (CULL && TREE_OK) {
+            usize n = V::len(self);
+            if (n > (usize)BOUNDS_CAP || n < TREE_MIN_ELEMENTS) return;
+            Bound *b = bounds_lds<OV>();
+            for (uint j = threadIdx.x; j < (uint)BOUNDS_CAP; j += blockDim.x) {
+                BoundS q = ht_scalar_bounds[j];
+                Bound x;
+                x.cx = (float)q.cx;
+                x.cy = (float)q.cy;
+                x.cz = (float)q.cz;
+                x.r = (float)q.r;
+                x.ri = q.r < 0.0 ? -1.f : (float)sqrt_(q.ri2);
+                x.horo = q.horo;
+                x.pad1 = q.pad;
+                x.pad2 = 0.f;
+                b[j] = x;
+            }
+        } else {
+            (void)self;
+        }
+#elif defined(HT_SCALAR_BOUNDS)
         (void)self;  // the bounds are in ht_scalar_bounds (fill_scalar)
 #else
         if constexpr (CULL) {
@@ -846,6 +892,66 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         }
     }
 
+#if defined(HT_GROUP_QUEUE)
+    // The two-level scan compacted at both levels (HT_GROUP_QUEUE).
+    // push_owned(element, candidate, ray lane) appends to the exact tests'
+    // ring, drain() runs its full batches.
+    template <class CR, class PushO, class Drain>
+    static HD void walk_groups_queued(const CR &cr, const ConstBoundS *sb, WaveScratch<G> *ws, uint lane,
+                                      unsigned long long below, uint nact, uint arank, PushO &push_owned,
+                                      Drain &drain) {
+        constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP, FIRST = P / GS;
+        static_assert(FIRST % CULL_UNROLL == 0, "level-1 steps cover the group nodes");
+        const Bound *leaves = bounds_lds<OV>();
+        uint gcount = 0, ghead = 0;
+        auto gpush = [&](uint k, bool pass) {
+            unsigned long long m = __ballot(pass);
+            if (m == 0) return;
+            if (pass) ws->gqueue[gring_wrap(ghead + gcount + (uint)__popcll(m & below))] = (k << 6) | lane;
+            gcount += (uint)__popcll(m);
+        };
+        // one batch of k <= nact (group, ray) pairs from the head of the ring
+        auto gbatch = [&](uint k) {
+            wave_sync();
+            bool have = arank < k;
+            uint gk = 0, L = 0;
+            if (have) {
+                uint e = ws->gqueue[gring_wrap(ghead + arank)];
+                gk = e >> 6;
+                L = e & 63u;
+            }
+            ghead = gring_wrap(ghead + k);
+            gcount -= k;
+            CR c2 = cr;
+            if (have) c2 = CR::make(ws->ray[L]);
+#pragma unroll
+            for (uint j = 0; j < GS; ++j) {
+                bool cand = false;
+                uint el = 0;
+                if (have) {
+                    const Bound &lb = leaves[gk * GS + j];
+                    el = __float_as_uint(lb.pad1);
+                    cand = el != NO_ELEMENT && c2.candidate(lb);
+                }
+                push_owned(el, cand, L);
+            }
+            drain();
+        };
+        for (uint k0 = 0; k0 < FIRST; k0 += CULL_UNROLL) {
+            bool pass[CULL_UNROLL];
+#pragma unroll
+            for (int u = 0; u < CULL_UNROLL; ++u) {
+                BoundS q = load_bound(sb + P + k0 + u);
+                pass[u] = q.r != -2.0 && cr.candidate(q);  // -2: a node over padding only
+            }
+#pragma unroll
+            for (int u = 0; u < CULL_UNROLL; ++u) gpush(k0 + u, pass[u]);
+            while (gcount >= nact) gbatch(nact);
+        }
+        if (gcount > 0) gbatch(gcount);
+    }
+#endif
+
 #if defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS)
     // walk_groups over the global array fill_scalar_block writes: every
     // node and leaf bound is one scalar load.
@@ -929,13 +1035,14 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             // a candidate; drain runs the exact tests of full batches (one
             // pair per active lane) from the head, once per scan step, so the
             // exact test's code appears once in the loop.
-            auto push = [&](usize i, bool cand) {
+            auto push_owned = [&](usize i, bool cand, uint owner) {
                 unsigned long long m = __ballot(cand);
                 if (m == 0) return;
                 if (cand)
-                    ws->queue[ring_wrap(qhead + qcount + (uint)__popcll(m & below))] = ((uint)i << 6) | lane;
+                    ws->queue[ring_wrap(qhead + qcount + (uint)__popcll(m & below))] = ((uint)i << 6) | owner;
                 qcount += (uint)__popcll(m);
             };
+            auto push = [&](usize i, bool cand) { push_owned(i, cand, lane); };
             auto drain = [&]() {
                 while (qcount >= nact) {
                     wave_sync();
@@ -952,7 +1059,9 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             bool tree = false;
             if constexpr (TREE_OK) tree = n >= TREE_MIN_ELEMENTS;
             if (tree) {
-#if defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS)
+#if defined(HT_GROUP_QUEUE)
+                if constexpr (TREE_OK) walk_groups_queued(cr, sb, ws, lane, below, nact, arank, push_owned, drain);
+#elif defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS)
                 if constexpr (TREE_OK) walk_groups_scalar(cr, sb, push, drain);
 #elif defined(HT_CULL_GROUPS)
                 if constexpr (TREE_OK) walk_groups(cr, push, drain);

@@ -19,7 +19,11 @@ frame with fewer than twice as many pixels as resident lanes loses);
 HT_WAVE_PRIO steps each wave's issue priority down as its slowest lane
 completes each quarter of the passes, so the waves with the most work left
 (the frame's critical path) issue first (with the pixel queue: once the
-queue is empty, by the work each wave's current pixels have left). Every
+queue is empty, by the work each wave's current pixels have left);
+HT_GROUP_QUEUE compacts the two-level scan at both levels: the (group,
+ray) pairs that pass a group sphere are queued and tested on full lanes
+against the group's leaves, so each ray pays for the groups it passes
+rather than for the groups any ray of its wave passes. Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -33,7 +37,7 @@ from typing import Dict, Optional, Sequence, Tuple
 
 from .processing import STRICT_FP, Canvas, Context, EntityBuffer, Render
 
-# (cull unroll, waves per SIMD, scalar bounds, group size or 0, pixel queue, wave priority)
+# (cull unroll, waves per SIMD, scalar bounds, group size or 0, pixel queue, wave priority, group queue)
 CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, False, 0, False), (4, 5, False, 0, False), (2, 5, False, 0, False), (4, 6, False, 0, False),
     (4, 4, True, 0, False), (4, 5, True, 0, False), (4, 6, True, 0, False), (4, 5, False, 4, False),
@@ -41,11 +45,13 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 4, False), (4, 6, True, 4, False), (4, 6, True, 8, False), (4, 6, True, 4, True),
     (4, 4, True, 4, True),
     (4, 4, True, 0, False, True), (4, 5, True, 0, False, True), (4, 6, True, 4, False, True),
-    (4, 4, True, 0, True, True), (4, 4, True, 4, True, True))
+    (4, 4, True, 0, True, True), (4, 4, True, 4, True, True),
+    (4, 4, True, 4, False, True, True), (4, 4, True, 8, False, True, True), (4, 6, True, 4, False, True, True),
+    (4, 4, True, 4, True, True, True), (4, 6, True, 4, True, True, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
-            pixel_queue: bool = False, wave_prio: bool = False) -> str:
+            pixel_queue: bool = False, wave_prio: bool = False, group_queue: bool = False) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
@@ -55,16 +61,19 @@ def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool 
         opts += " -DHT_PIXEL_QUEUE=1"
     if wave_prio:
         opts += " -DHT_WAVE_PRIO=1"
+    if group_queue:
+        opts += " -DHT_GROUP_QUEUE=1"
     return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool]:
     unroll, waves = int(candidate[0]), int(candidate[1])
     sb = bool(candidate[2]) if len(candidate) > 2 else False
     groups = int(candidate[3]) if len(candidate) > 3 else 0
     queue = bool(candidate[4]) if len(candidate) > 4 else False
     prio = bool(candidate[5]) if len(candidate) > 5 else False
-    return unroll, waves, sb, groups, queue, prio
+    gq = bool(candidate[6]) if len(candidate) > 6 else False
+    return unroll, waves, sb, groups, queue, prio, gq
 
 
 def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
@@ -76,7 +85,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
                 candidates: Sequence[Sequence] = CANDIDATES, reps: int = 3, base: str = STRICT_FP,
                 cache_path: Optional[str] = None) -> Dict:
     """Times `spp` passes of render_passes (HIP events, median of `reps`) for
-    each (unroll, waves[, scalar bounds[, groups[, pixel queue[, wave priority]]]]) candidate; returns {"best": {...},
+    each (unroll, waves[, scalar bounds[, groups[, pixel queue[, wave priority[, group queue]]]]]) candidate;
+    returns {"best": {...},
     "table": [...]} with the program options of the best."""
     key = f"gfx950:{scene.cpp()}:{width}x{height}:{spp}:{base}"
     if cache_path and os.path.exists(cache_path):
@@ -87,8 +97,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb, groups, queue, prio = _knobs(cand)
-        opts = options(unroll, waves, base, sb, groups, queue, prio)
+        unroll, waves, sb, groups, queue, prio, gq = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups, queue, prio, gq)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -99,7 +109,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             r.render_passes(buf, c, spp)
             times.append(ctx.timer_end())
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                      "pixel_queue": queue, "wave_prio": prio, "ms": sorted(times)[len(times) // 2], "options": opts})
+                      "pixel_queue": queue, "wave_prio": prio, "group_queue": gq,
+                      "ms": sorted(times)[len(times) // 2], "options": opts})
     best = min(table, key=lambda t: t["ms"])
     result = {"best": best, "table": table}
     if cache_path:

@@ -81,10 +81,12 @@ def render(spec: str, w: int, h: int, passes: int, seeds: np.ndarray, canvas: np
     return canvas, seeds, int(flops.value)
 
 
-def count_flops(spec: str, w: int, h: int, passes: int, seeds: np.ndarray, culled: bool, crop=None,
+def count_flops(spec: str, w: int, h: int, passes: int, seeds: np.ndarray, culled, crop=None,
                 threads: int = 0):
     """FP32 render of the crop on the counting scalars. Returns (canvas,
-    seeds, fp32 flops, fp64 flops); culled=True counts the culled scan's
+    seeds, fp32 flops, fp64 flops); culled: 0 / False = the reference's
+    plain scan, 1 / True = the flat culled scan, g = the two-level scan over
+    groups of g, 1000 + g = the same with the group queue. culled=True counts the culled scan's
     algorithm (FP64 cull tests, exact tests of the candidates only)."""
     seeds = np.array(seeds, dtype=np.uint32).reshape(h, w).copy()
     canvas = np.zeros((h, w, 4), np.float32)

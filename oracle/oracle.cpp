@@ -28,6 +28,7 @@
 // (sphere tracing, roulette), tests/test_oracle_scenes.py (whole paths
 // against closed forms). Every function here has such a check.
 #include <atomic>
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -909,6 +910,12 @@ template <class R, class F, class G> struct MappedObject : Object<R, F, G> {
 // hit as the reference loop below.
 thread_local uint64_t g_flops64 = 0;
 bool g_count_culled = false;
+// The two-level scan's count (program options HT_SCALAR_BOUNDS with
+// HT_CULL_GROUPS = g_count_group): the group spheres' tests, then the leaf
+// tests of the groups the ray passes; with g_count_queue (HT_GROUP_QUEUE)
+// also the rebuild of the ray's cull state for each (group, ray) pair.
+int g_count_group = 0;
+bool g_count_queue = false;
 
 struct CntD {
     double v;
@@ -924,6 +931,7 @@ inline CntD sqrtd(CntD a) { ++g_flops64; return CntD(std::sqrt(a.v)); }
 struct BoundF {
     float cx = 0, cy = 0, cz = 0, r = -1, ri = -1;
     bool horo = false;
+    bool horo_node = false;  // a tree node over a horoball: no "behind" cull
 };
 
 inline float next_up(float f) { uint32_t u; std::memcpy(&u, &f, 4); ++u; std::memcpy(&f, &u, 4); return f; }
@@ -1045,7 +1053,8 @@ struct CullRayHy {
         CntD d2 = ex * ex + CntD(b.cz) * CntD(b.cz);
         bool cull = rp2.v < 0.0;
         bool behind = sig.v < 0.0 && (sig * sig).v > rp2.v;
-        if (behind && b.horo) behind = !toward_tangent_point(b.cx, b.cy);
+        if (behind && b.horo_node) behind = false;
+        else if (behind && b.horo) behind = !toward_tangent_point(b.cx, b.cy);
         cull = cull || behind;
         CntD o = d2 - rg2 - rp2;
         CntD four_rr = four_rg2 * rp2;
@@ -1076,10 +1085,149 @@ template <class R, class F, class G> struct VecObject : Object<R, F, G> {
     std::vector<BoundF> bounds;
     bool cullable = false;
 
+    // the group tree of the device's bounds_fill (scan.hh build_tree /
+    // fill_scalar_block): leaves in Morton order of the FP32 bound centres
+    // (element index order among equal codes, unbounded elements after all
+    // others, padding last) over P = the program's bounds capacity, nodes
+    // merged bottom-up in FP64 and rounded outwards to FP32
+    size_t cap = 0;
+    std::vector<uint32_t> order;   // leaf j -> element index, 0xFFFF = padding
+    std::vector<BoundF> group;     // the group nodes covering g_count_group leaves each
+    std::vector<bool> group_empty; // a node over padding only
+
+    static uint32_t expand10(uint32_t v) {
+        v &= 1023u;
+        v = (v | (v << 16)) & 0x030000FFu;
+        v = (v | (v << 8)) & 0x0300F00Fu;
+        v = (v | (v << 4)) & 0x030C30C3u;
+        v = (v | (v << 2)) & 0x09249249u;
+        return v;
+    }
+    static V4<float> merge_spheres(V4<float> a, V4<float> b) {
+        double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
+        double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+        double cx, cy, cz, r;
+        if (d + b.w <= a.w) {
+            cx = a.x, cy = a.y, cz = a.z, r = a.w;
+        } else if (d + a.w <= b.w) {
+            cx = b.x, cy = b.y, cz = b.z, r = b.w;
+        } else {
+            r = 0.5 * (d + (double)a.w + (double)b.w);
+            double t = (r - a.w) / d;
+            cx = a.x + dx * t, cy = a.y + dy * t, cz = a.z + dz * t;
+        }
+        float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+        double err = std::fabs((double)fx - cx) + std::fabs((double)fy - cy) + std::fabs((double)fz - cz);
+        double ro = r * (1.0 + 1e-6) + err + 1e-30;
+        float fr = (float)ro;
+        if ((double)fr < ro) fr = next_up(fr);
+        return {fx, fy, fz, fr};
+    }
+    void build_groups(int g) {
+        size_t n = items.size();
+        cap = 64;
+        while (cap < n) cap *= 2;
+        order.clear();
+        group.clear();
+        group_empty.clear();
+        if (g <= 0 || cap > 1024 || n < 16) return;  // the device's flat scan
+        const size_t P = cap;
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (auto &b : bounds) {
+            if (b.r < 0) continue;
+            float c[3] = {b.cx, b.cy, b.cz};
+            for (int a = 0; a < 3; ++a) lo[a] = std::fmin(lo[a], c[a]), hi[a] = std::fmax(hi[a], c[a]);
+        }
+        std::vector<uint64_t> key(P, ~0ull);
+        for (size_t i = 0; i < n; ++i) {
+            uint32_t code = 0xFFFFFFFFu;
+            if (bounds[i].r >= 0) {
+                float c[3] = {bounds[i].cx, bounds[i].cy, bounds[i].cz};
+                code = 0;
+                for (int a = 0; a < 3; ++a) {
+                    float u = hi[a] > lo[a] ? (c[a] - lo[a]) / (hi[a] - lo[a]) : 0.5f;
+                    uint32_t q = (uint32_t)std::fmin(std::fmax(u * 1024.0f, 0.0f), 1023.0f);
+                    code |= expand10(q) << (2 - a);
+                }
+            }
+            key[i] = ((uint64_t)code << 32) | i;
+        }
+        std::sort(key.begin(), key.end());
+        order.resize(P);
+        for (size_t j = 0; j < P; ++j) {
+            uint32_t low = (uint32_t)(key[j] & 0xFFFFFFFFull);
+            order[j] = low < n ? low : 0xFFFFu;
+        }
+        // nodes k = 1 .. P - 1 of the implicit tree, children 2k, 2k + 1
+        std::vector<V4<float>> node(P);
+        std::vector<char> nh(P, 0);
+        for (size_t first = P / 2; first >= 1; first >>= 1) {
+            for (size_t k = first; k < 2 * first; ++k) {
+                V4<float> sp[2];
+                bool h[2];
+                for (int c = 0; c < 2; ++c) {
+                    size_t ch = 2 * k + c;
+                    if (ch >= P) {
+                        uint32_t e = order[ch - P];
+                        if (e == 0xFFFFu) {
+                            sp[c] = {0.f, 0.f, 0.f, -2.f};
+                            h[c] = false;
+                        } else {
+                            const BoundF &bb = bounds[e];
+                            sp[c] = bb.r < 0 ? V4<float>{0.f, 0.f, 0.f, -1.f} : V4<float>{bb.cx, bb.cy, bb.cz, bb.r};
+                            h[c] = bb.horo;
+                        }
+                    } else {
+                        sp[c] = node[ch];
+                        h[c] = nh[ch] != 0;
+                    }
+                }
+                V4<float> m;
+                if (sp[0].w == -2.f) m = sp[1];
+                else if (sp[1].w == -2.f) m = sp[0];
+                else if (sp[0].w < 0.f || sp[1].w < 0.f) m = {0.f, 0.f, 0.f, -1.f};
+                else m = merge_spheres(sp[0], sp[1]);
+                node[k] = m;
+                nh[k] = (h[0] || h[1]) ? 1 : 0;
+            }
+            if (first == 1) break;
+        }
+        size_t FIRST = P / (size_t)g;
+        for (size_t k = FIRST; k < 2 * FIRST; ++k) {
+            BoundF q;
+            q.cx = node[k].x, q.cy = node[k].y, q.cz = node[k].z, q.r = node[k].w;
+            q.ri = 0.f;
+            q.horo_node = nh[k] != 0;
+            group.push_back(q);
+            group_empty.push_back(node[k].w == -2.f);
+        }
+    }
+
     void prepare_cull() {
         bounds.assign(items.size(), BoundF());
         cullable = !items.empty() && items.size() <= 4096;
         for (size_t i = 0; i < items.size() && cullable; ++i) cullable = element_bound(items[i].get(), bounds[i]);
+        if (cullable && g_count_group > 0) build_groups(g_count_group);
+    }
+
+    // the candidate elements of a ray, in the order the scan finds them
+    template <class CR, class Ray_> void candidates(const CR &cr, const Ray_ &ray, std::vector<uint32_t> &out) const {
+        out.clear();
+        if (group.empty()) {
+            for (size_t i = 0; i < items.size(); ++i)
+                if (cr.candidate(bounds[i])) out.push_back((uint32_t)i);
+            return;
+        }
+        const size_t g = (size_t)g_count_group;
+        for (size_t k = 0; k < group.size(); ++k) {
+            if (group_empty[k] || !cr.candidate(group[k])) continue;
+            CR c2 = cr;
+            if (g_count_queue) c2 = cull_ray(ray);  // the pair's lane rebuilds the ray's cull state
+            for (size_t j = 0; j < g; ++j) {
+                uint32_t e = order[k * g + j];
+                if (e != 0xFFFFu && c2.candidate(bounds[e])) out.push_back(e);
+            }
+        }
     }
 
     // The culled scan's work: cull tests, exact tests of the candidates
@@ -1090,8 +1238,10 @@ template <class R, class F, class G> struct VecObject : Object<R, F, G> {
         auto cr = cull_ray(light.ray);
         R min_dist = R(-1);
         uint32_t min_index = 0;
-        for (size_t i = 0; i < items.size(); ++i) {
-            if (!cr.candidate(bounds[i])) continue;
+        std::vector<uint32_t> cand;
+        candidates(cr, light.ray, cand);
+        std::sort(cand.begin(), cand.end());  // the reference loop's order decides ties
+        for (uint32_t i : cand) {
             auto mo = static_cast<const MappedObject<R, F, G> *>(items[i].get());
             Light<R, F, G> cur = light;
             Cache<R, G> cc = cache;
@@ -1641,21 +1791,30 @@ int oracle_render(const char *spec, int precision, uint32_t w, uint32_t h, uint3
 // Flop count of the FP32 render over a crop: *flops32 = FP32 flops, *flops64
 // = FP64 flops. culled = 0: the reference algorithm (every element's exact
 // test, each mapped back); culled = 1: the culled scan's algorithm (FP64
-// cull tests, exact tests of candidates, one map back per hit).
+// cull tests, exact tests of candidates, one map back per hit); culled = g
+// (2..16): the two-level scan over groups of g (group-sphere tests, then
+// the leaf tests of the passing groups); culled = 1000 + g: the same with
+// the group queue (the ray's cull state rebuilt per passing group).
 int oracle_count_flops(const char *spec, int culled, uint32_t w, uint32_t h, uint32_t passes, float *canvas,
                        uint32_t *seeds, uint32_t x0, uint32_t y0, uint32_t cw, uint32_t ch, int threads,
                        uint64_t *flops32, uint64_t *flops64) {
     try {
         if (x0 + cw > w || y0 + ch > h) throw std::runtime_error("crop outside the image");
         g_count_culled = culled != 0;
+        g_count_queue = culled >= 1000;
+        g_count_group = culled >= 2 ? culled % 1000 : 0;
         uint64_t f64 = 0;
         uint64_t f32 = render_any<Cnt, Cnt>(spec, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads, &f64);
         g_count_culled = false;
+        g_count_group = 0;
+        g_count_queue = false;
         if (flops32) *flops32 = f32;
         if (flops64) *flops64 = f64;
         return 0;
     } catch (const std::exception &e) {
         g_count_culled = false;
+        g_count_group = 0;
+        g_count_queue = false;
         g_err = e.what();
         return -1;
     }

@@ -226,6 +226,11 @@ CULLED_VARIANTS = {
     "prio": STRICT_FP + " -DHT_WAVE_PRIO=1",
     "sb_q_prio": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_PIXEL_QUEUE=1 -DHT_WAVE_PRIO=1",
     "sb_g4_q_prio": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_PIXEL_QUEUE=1 -DHT_WAVE_PRIO=1",
+    "gq4_prio": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_WAVE_PRIO=1",
+    "gq8": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=8 -DHT_GROUP_QUEUE=1",
+    "gq4_q_prio_w6": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_PIXEL_QUEUE=1"
+                     " -DHT_WAVE_PRIO=1 -DHT_WAVES_PER_EU=6",
+    "gq16_u2": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=16 -DHT_GROUP_QUEUE=1 -DHT_CULL_UNROLL=2",
 }
 
 
@@ -242,7 +247,11 @@ CULLED_VARIANTS = {
     ("hy_cells", False, "sb_groups8_w6_q"), ("sdf_1024", False, "sb_groups8_w6_q"),
     ("hy_example", False, "sb_groups8_w6_q"), ("hy_cells", True, "sb_prio_w5"), ("sdf_1024", False, "sb_prio_w5"),
     ("eu_ball", False, "prio"), ("hy_example", False, "prio"), ("hy_cells", True, "sb_q_prio"),
-    ("sdf_1024", False, "sb_g4_q_prio"), ("hy_cells", False, "sb_g4_q_prio")])
+    ("sdf_1024", False, "sb_g4_q_prio"), ("hy_cells", False, "sb_g4_q_prio"),
+    ("hy_cells", True, "gq4_prio"), ("hy_cells", False, "gq8"), ("sdf_1024", False, "gq4_prio"),
+    ("sdf_1024", False, "gq8"), ("eu_ball", False, "gq4_prio"), ("hy_example", False, "gq8"),
+    ("eu_example", False, "gq4_prio"), ("hy_cells", True, "gq4_q_prio_w6"), ("sdf_1024", False, "gq4_q_prio_w6"),
+    ("hy_cells", False, "gq16_u2"), ("sdf", False, "gq8")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
@@ -262,10 +271,10 @@ def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio"])
+@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6"])
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
 def test_pixel_queue_ragged_frames(ctx, size, variant):
-    """The pixel-queue program on frames smaller than a block or with a
+    """The pixel-queue programs on frames smaller than a block or with a
     partial last wave, launched twice (the counter is reset before every
     launch): the same canvas and seeds as the one-thread-per-pixel kernel."""
     w, h = size

@@ -169,3 +169,14 @@ def test_culled_flop_count_keeps_the_render():
         c1, s1, c32, c64 = oracle.count_flops(spec, 64, 48, 2, seeds, True)
         assert np.array_equal(c0, c1) and np.array_equal(s0, s1), wl.name
         assert p64 == 0 and c64 > 0 and c32 < p32, (wl.name, p32, c32, c64)
+        # the two-level scan (groups of g; 1000 + g with the group queue):
+        # the same render, the same exact tests, fewer cull flops than the
+        # flat scan where the scene has the tree's 16 elements
+        for mode in (4, 8, 1004, 1008):
+            cg, sg, g32, g64 = oracle.count_flops(spec, 64, 48, 2, seeds, mode)
+            assert np.array_equal(c0, cg) and np.array_equal(s0, sg), (wl.name, mode)
+            assert g32 == c32, (wl.name, mode)
+            if len(wl.scene.object.items) >= 16:
+                assert 0 < g64 < c64, (wl.name, mode, g64, c64)
+            else:
+                assert g64 == c64, (wl.name, mode)
