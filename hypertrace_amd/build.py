@@ -176,6 +176,9 @@ def _suite_jobs():
             jobs.append((s, X32_F32, opts, codegen.root_source(s, X32_F32, wavefront=True)))
         s = scenes.hyperbolic_cells(80, 60).scene
         jobs += [(s, X32_F64, None, None), (s, X32_F64, None, codegen.root_source(s, X32_F64, wavefront=True))]
+    pl = load("plugin_scenes")
+    if pl is not None:  # the plugin scene and its built-in twin (test_gpu_plugins.py)
+        jobs += [(s, cfg, None, None) for s in pl.scene_pair() for cfg in (X32_F32, X32_F64)]
     # bench.py legs
     cells = scenes.hyperbolic_cells().scene
     wf_src = codegen.root_source(cells, X32_F32, wavefront=True)

@@ -14,20 +14,79 @@ import glob
 import os
 from typing import Dict, List, Tuple
 
-from .objects import Covered, Entity, Mapped, ObjectVec, SceneImpl, _ChoiceBase, _Component, _Tiled, _VecBase
+from .objects import (Covered, Entity, Mapped, ObjectVec, Plugin, SceneImpl, _ChoiceBase, _Component, _Tiled,
+                      _VecBase)
 from .types import Config
 
 DEVICE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "device")
 
+# Include names of the reference's device library (kernel/ocl/...) that a
+# plugin's sources may include. The plugin region (ht_plugin/begin.hh)
+# already provides their vocabulary, so each maps to an empty header.
+REFERENCE_INCLUDES = (
+    "macros.hh", "types.hh", "random.hh", "hash.hh",
+    "algebra/real.hh", "algebra/vector.hh", "algebra/color.hh", "algebra/complex.hh", "algebra/quaternion.hh",
+    "render/context.hh", "render/light/base.hh", "render/light/eu.hh", "render/light/hy.hh",
+    "geometry/euclidean.hh", "geometry/hyperbolic.hh", "geometry/eu.hh", "geometry/hy.hh",
+    "geometry/ray_eu.hh", "geometry/ray_hy.hh", "transform/rotation.hh",
+    "shape/interface.hh", "object/interface.hh", "material/interface.hh", "background/interface.hh",
+    "view/interface.hh",
+)
 
-def device_headers() -> Dict[str, str]:
-    """Device library files keyed by include name ("ht/base.hh", ...)."""
+
+def _read_dir(sub: str) -> Dict[str, str]:
     out = {}
-    for path in sorted(glob.glob(os.path.join(DEVICE_DIR, "ht", "*.hh"))):
+    for path in sorted(glob.glob(os.path.join(DEVICE_DIR, sub, "*.hh"))):
         name = os.path.relpath(path, DEVICE_DIR)
         with open(path) as f:
             out[name] = f.read()
     return out
+
+
+def device_headers() -> Dict[str, str]:
+    """Device library files keyed by include name ("ht/base.hh", ...)."""
+    return _read_dir("ht")
+
+
+def plugins(*scenes: Entity) -> List[Plugin]:
+    """The plugin entities of the scenes, each type once, in tree order."""
+    out: List[Plugin] = []
+    seen = set()
+    stack: List[Entity] = list(reversed(scenes))
+    while stack:
+        e = stack.pop()
+        if isinstance(e, Plugin) and e.prefix not in seen:
+            seen.add(e.prefix)
+            out.append(e)
+        stack.extend(reversed(_children(e)))
+    return out
+
+
+def program_headers(*scenes: Entity) -> Dict[str, str]:
+    """Header map of a program over `scenes`: the device library, plus, when
+    a scene holds plugin entities, the plugin region, the reference's include
+    names it stands in for and the plugins' own sources."""
+    out = device_headers()
+    found = plugins(*[s for s in scenes if s is not None])
+    if not found:
+        return out
+    out.update(_read_dir("ht_plugin"))
+    for name in REFERENCE_INCLUDES:
+        out[name] = "#pragma once\n// Provided by the plugin region (ht_plugin/begin.hh).\n"
+    for p in found:
+        for name, text in p.sources.items():
+            if name in out and out[name] != text:
+                raise ValueError(f"plugin {p.prefix}: source {name!r} clashes with another file of that name")
+            out[name] = text
+    return out
+
+
+def _plugin_includes(*scenes: Entity) -> List[str]:
+    found = plugins(*scenes)
+    if not found:
+        return []
+    return (["#include <ht_plugin/begin.hh>"] + [f"#include <{h}>" for h in dict.fromkeys(p.header for p in found)]
+            + ["#include <ht_plugin/end.hh>"])
 
 
 def _children(e: Entity) -> List[Entity]:
@@ -41,6 +100,8 @@ def _children(e: Entity) -> List[Entity]:
         return [e.map, e.inner]
     if isinstance(e, _Tiled):
         return list(e.materials) + [e.border_material]
+    if isinstance(e, Plugin):
+        return []
     kids = []
     for name in ("inner", "shape", "material"):
         if hasattr(e, name) and isinstance(getattr(e, name), Entity):
@@ -98,6 +159,7 @@ def root_source(scene: SceneImpl, cfg: Config, wavefront: bool = False) -> str:
     ]
     if wavefront:
         lines.append("#include <ht/wavefront.hh>")
+    lines += _plugin_includes(scene)
     lines.append(f"typedef {scene.cpp()} Scene;")
     lines += layout_asserts(scene, cfg)
     lines.append("HT_DEFINE_RENDER_KERNELS(Scene)")
@@ -125,6 +187,9 @@ def batch_source(scene0: SceneImpl, scene1: SceneImpl, cfg: Config, block: int =
         f"#define HT_MAX_WAVES {block // 64}",
         f"#define HT_BATCH_BLOCK {block}",
         "#include <ht/batch.hh>",
+    ]
+    lines += _plugin_includes(scene0, scene1)
+    lines += [
         f"typedef {scene0.cpp()} Scene0;",
         f"typedef {scene1.cpp()} Scene1;",
     ]

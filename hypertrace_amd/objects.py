@@ -873,6 +873,66 @@ class GradBg(Entity):
                 + [_h32(x) for x in self.colors[1]] + [_h32(self.power)])
 
 
+# --------------------------------------------------------------------- plugins
+
+PLUGIN_KINDS = ("shape", "object", "material", "background", "view")
+
+_FIELD_KINDS = {
+    "real": _Real,      # real (f32 or f64 with the program's precision)
+    "float": _F32,      # f32
+    "float3": _Color,   # Vector<f32, 3>: 16 B
+    "real3": _Real3,    # Vector<real, 3>: 4 reals
+    "real2": _RealN,    # Vector<real, N>, N = 2, 4, 8, 16: N reals
+    "real4": _RealN,
+}
+
+
+def plugin_field(kind: str, value):
+    """A field of a plugin entity's C struct, in declaration order: `kind`
+    is its device type name ("real", "float", "float3", "real3", "real2",
+    "real4"), as the reference's derive names the field types
+    (types/src/primitive/vector.rs:16-19, scalar.rs)."""
+    if kind not in _FIELD_KINDS:
+        raise ValueError(f"plugin field kind must be one of {sorted(_FIELD_KINDS)}")
+    return _FIELD_KINDS[kind](value)
+
+
+class Plugin(Entity):
+    """An entity whose device code is a plugin written against the
+    reference's device interface: a C struct of its fields and free
+    functions named `<prefix>_detect` / `<prefix>_interact` /
+    `<prefix>_sample`, declared by `define_<kind>_interface(Self, prefix,
+    ...)` in the header `header` (the reference's user-entity seam,
+    kernel/ocl/{shape,object,material,background,view}/interface.hh).
+    `sources` maps include names to file texts (the header, the files it
+    includes); `fields` are `plugin_field`s in declaration order, which fix
+    the bytes as the reference's derive does (type-macros/src/entity.rs).
+    The device type is the adaptor `ht_plugin::<prefix>` the interface macro
+    defines. The oracle has no counterpart: a plugin runs on the device only."""
+
+    def __init__(self, kind: str, prefix: str, header: str, sources, fields: Sequence = ()):
+        if kind not in PLUGIN_KINDS:
+            raise ValueError(f"plugin kind must be one of {PLUGIN_KINDS}")
+        if not prefix.isidentifier():
+            raise ValueError("plugin prefix must be a C identifier")
+        self.kind = kind
+        self.prefix = prefix
+        self.header = header
+        self.sources = dict(sources)
+        if header not in self.sources:
+            raise ValueError(f"plugin sources lack the header {header!r}")
+        self._fields = list(fields)
+
+    def fields(self):
+        return self._fields
+
+    def cpp(self, geo):
+        return f"ht_plugin::{self.prefix}"
+
+    def spec(self, geo, cfg):
+        raise NotImplementedError("plugin entities have no oracle counterpart")
+
+
 # --------------------------------------------------------------------- scene
 
 class SceneImpl(Entity):

@@ -206,7 +206,7 @@ class Program:
         self.ctx = ctx
         self.fp_options = fp_options
         self.source = source if source is not None else codegen.root_source(scene, ctx.config)
-        hdrs = codegen.device_headers()
+        hdrs = codegen.program_headers(scene)
         names = native.cstr_array(list(hdrs.keys()))
         contents = native.cstr_array(list(hdrs.values()))
         h = c_void_p()
@@ -269,7 +269,7 @@ def precompile(scene: Optional[SceneImpl], config: Config, name: str = "render.h
     """Compile a scene's program (or the given root source) into the
     code-object cache (no GPU needed). Returns True if it was already cached."""
     source = source if source is not None else codegen.root_source(scene, config)
-    hdrs = codegen.device_headers()
+    hdrs = codegen.program_headers(scene)
     lib = native.lib()
     native.check(lib.ht_prog_precompile(name.encode(), source.encode(), len(hdrs),
                                         native.cstr_array(list(hdrs.keys())),
