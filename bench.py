@@ -179,7 +179,8 @@ def summary(out, detail_path=None):
         legs["batch_32_jobs"] = (
             {"graph": bt["graph_replay"]["value"], "eager": bt["eager"]["value"],
              "one_job_per_launch": bt["one_job_per_launch"]["value"],
-             "bit_identical": bt.get("bit_identical_to_single_jobs")}
+             "bit_identical": bt.get("bit_identical_to_single_jobs"),
+             "small_jobs_graph_speedup": (bt.get("small_jobs") or {}).get("graph_speedup")}
             if "eager" in bt else {"error": str(bt.get("error"))[:120]})
     wf = out.get("wavefront_cells")
     if wf:
@@ -425,7 +426,7 @@ def cache_leg(args):
     from hypertrace_amd import scenes
     from hypertrace_amd.maps import compose_on_device
     from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
-    from hypertrace_amd.radiance_cache import CachedRender, RadianceCache
+    from hypertrace_amd.radiance_cache import CachedRender, RadianceCache, init_params
     from hypertrace_amd.types import X32_F32
 
     wl = scenes.hyperbolic_cells(1920, 1080, 1, primitives=256)
@@ -465,6 +466,35 @@ def cache_leg(args):
                 reached = spp
                 break
         queries, records = cache.counts()
+        # the same method with the paths ending into the cache at their 2nd
+        # and 4th hit: RMSE curves over a fixed number of passes, for the
+        # bias / variance split of each
+        hops = {}
+        for qh in (1, 3):
+            crh = CachedRender(ctx, wl.scene, cache, query_hop=qh, train_every=16, steps_per_pass=4,
+                               batch=16384, lr=1e-3, fp_options=cached_knobs["options"])
+            cache.set_params(init_params(0))  # a fresh network for each query hop
+            ch = Canvas(ctx, (W, H), wl.seed_mode, wl.seed)
+            th, sh, cv = 0.0, 0, []
+            while sh < args.cache_split_spp:
+                ctx.sync()
+                t0 = time.perf_counter()
+                for _ in range(8):
+                    crh.render_pass(buf, ch)
+                ctx.sync()
+                th += time.perf_counter() - t0
+                sh += 8
+                cv.append((sh, round(rmse(gamma_image(ch.load_image()), ref), 6)))
+            fit = bias_variance(cv)
+            hops[f"query_hop_{qh}"] = {"ms_per_pass": th * 1e3 / sh, "curve": cv[::4] + cv[-1:], "fit": fit,
+                                       "spp_for_target": spp_for(fit, args.target_rmse)}
+    chosen_fit = bias_variance(base.get("curve") or [])
+    cached_fit = bias_variance(curve)
+    split = {"chosen": {"fit": chosen_fit, "spp_for_target": spp_for(chosen_fit, args.target_rmse),
+                        "ms_per_pass": (base["chosen_s"] * 1e3 / base["spp"]) if base.get("spp") else None},
+             "query_hop_2": {"fit": cached_fit, "spp_for_target": spp_for(cached_fit, args.target_rmse),
+                             "ms_per_pass": t * 1e3 / max(spp, 1)}}
+    split.update(hops)
     out = {"workload": "hyperbolic_cells 1920x1080, 256 primitives, 8 hops", "target": args.target_rmse,
            "oracle_spp": reference[3], "oracle_noise_rmse": reference[1], "oracle_render_s": reference[2],
            "bit_exact_forms": {k: base.get(k) for k in ("reached", "spp", "chosen_s", "plain_s", "speedup")},
@@ -473,12 +503,35 @@ def cache_leg(args):
                       "seconds": t, "ms_per_pass": t * 1e3 / max(spp, 1), "curve": curve[::4] + curve[-1:],
                       "queries_last_pass": queries, "training_records_last_pass": records,
                       "method": "paths end into the MLP cache at their third hit; 1/16 of pixels per pass "
-                                "run unbiased to full depth and train it (4 Adam steps x 16384 per pass)"}}
+                                "run unbiased to full depth and train it (4 Adam steps x 16384 per pass)"},
+           "bias_variance": split}
     if reached is not None and base.get("plain_s"):
         out["cached"]["speedup_vs_plain"] = base["plain_s"] / t
     if reached is not None and base.get("chosen_s"):
         out["cached"]["speedup_vs_chosen"] = base["chosen_s"] / t
     return out
+
+
+def bias_variance(curve, min_spp=8):
+    """Least-squares fit of RMSE(spp)^2 = bias^2 + var / spp to the points
+    of an RMSE curve with spp >= min_spp: the error floor no number of
+    samples removes (the method's bias plus the reference image's own
+    noise) and the per-sample variance. Returns the two terms and the spp
+    the fit needs for a target (None when the floor is above it)."""
+    pts = [(float(n), float(e)) for n, e in curve if n >= min_spp]
+    if len(pts) < 2:
+        return None
+    a = np.array([[1.0, 1.0 / n] for n, _ in pts])
+    y = np.array([e * e for _, e in pts])
+    (b2, v), *_ = np.linalg.lstsq(a, y, rcond=None)
+    b2, v = max(float(b2), 0.0), max(float(v), 0.0)
+    return {"floor_rmse": b2 ** 0.5, "var_per_spp": v, "points": len(pts)}
+
+
+def spp_for(fit, target):
+    if fit is None or fit["floor_rmse"] >= target:
+        return None
+    return fit["var_per_spp"] / (target * target - fit["floor_rmse"] ** 2)
 
 
 def fixed_knobs(text, base=None):
@@ -667,6 +720,25 @@ def batch_leg(args):
                                  and np.array_equal(cb.load_seeds(), c1.load_seeds()))
             one.close()
         fresh.close()
+        # many tiny jobs (256 thumbnails of 16x16..32x32, 1 spp, 8..16
+        # primitives) with the same program: the frame's 2 x 256 + 3 launches
+        # (a canvas clear and a post pass per job, the counter reset, the
+        # bounds fill, the persistent kernel) take the host longer to enqueue
+        # than the device to run, which is where one hipGraph replay pays
+        small = scenes.batch_jobs(256, (16, 16), (32, 32), (1, 1), (4, 6), (8, 16))
+        small_paths = sum(s.width * s.height * s.spp for s in small)
+        sbatch = Batch(ctx, bprog, small, best["tile"], with_rgba8=True)
+        small_eager_s = timed(lambda: sbatch.render(grid, clear=True, post=True), 20)
+        sbatch.capture(grid, clear=True, post=True)
+        small_graph_s = timed(sbatch.replay, 20)
+        sbatch.close()
+    small_out = {"workload": "256 jobs, 16x16..32x32, 1 spp, depth 4..6, 8..16 primitives, post pass per job",
+                 "paths_per_frame": small_paths,
+                 "eager": {"ms_per_frame": small_eager_s * 1e3, "value": small_paths / small_eager_s / 1e6,
+                           "unit": "Mpaths/s"},
+                 "graph_replay": {"ms_per_frame": small_graph_s * 1e3, "value": small_paths / small_graph_s / 1e6,
+                                  "unit": "Mpaths/s"},
+                 "graph_speedup": small_eager_s / small_graph_s}
     return {"workload": "32 jobs, half Euclidean / half hyperbolic, 128x128..1024x768, spp 1..64, depth 4..12, "
                         "8..256 primitives, diffuse/mirror/glass 0.6/0.25/0.15",
             "paths_per_frame": paths, "autotuned": best,
@@ -675,6 +747,7 @@ def batch_leg(args):
             "one_job_per_launch": {"ms_per_frame": single_s * 1e3, "value": paths / single_s / 1e6,
                                    "unit": "Mpaths/s", "program": solo},
             "bit_identical_to_single_jobs": same,
+            "small_jobs": small_out,
             "tuner_table": tune["table"]}
 
 
@@ -687,6 +760,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="time only the chosen form")
     ap.add_argument("--no-cache", action="store_true", help="skip the 1080p radiance-cache leg")
     ap.add_argument("--cache-max-spp", type=int, default=1024)
+    ap.add_argument("--cache-split-spp", type=int, default=256,
+                    help="passes of the query-hop 1 and 3 runs of the radiance-cache leg")
     ap.add_argument("--no-sdf", action="store_true", help="skip the 1080p SDF leg")
     ap.add_argument("--no-euclidean", action="store_true", help="skip the 320x240 Euclidean leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batched leg")

@@ -58,16 +58,20 @@ def standard_scenes():
     ]
 
 
-def precompile_programs(verbose: bool = False) -> None:
-    from .processing import FAST_FP, NO_CULL_FP, precompile, program_options
-    from .types import X32_F32, X32_F64
-
+def _standard_jobs():
+    """(scene, config, fp options, None) of the standard programs: per
+    standard scene the FP32 / FP64 / fast / element-by-element programs,
+    Russian roulette for the Euclidean ones, the sphere-tree option, the
+    wider FP32 hit window, and the render tuner's candidates for the cell
+    frame and (with roulette) the 1024-shape SDF scene."""
+    from .processing import FAST_FP, NO_CULL_FP, program_options
     from .tuning import CANDIDATES, candidate_options
+    from .types import X32_F32, X32_F64
 
     rr = program_options(rr_depth=3)
     tree = program_options() + " -DHT_CULL_TREE=1"
-    scenes_ = standard_scenes()
-    for k, scene in enumerate(scenes_):
+    jobs = []
+    for k, scene in enumerate(standard_scenes()):
         extra = [(X32_F32, rr), (X32_F64, rr)] if scene.geo == "Eu" else []
         if k in (2, 5):  # the cell scene and the 1024-shape scene: the sphere-tree option
             extra.append((X32_F32, tree))
@@ -79,11 +83,8 @@ def precompile_programs(verbose: bool = False) -> None:
         if k == 5:  # the 1024-shape scene: the tuner's candidates with Russian roulette
             extra += [(X32_F32, candidate_options(c, rr)) for c in CANDIDATES]
         for cfg, fp in [(X32_F32, None), (X32_F64, None), (X32_F32, FAST_FP), (X32_F32, NO_CULL_FP)] + extra:
-            t = time.time()
-            hit = precompile(scene, cfg, fp_options=fp)
-            if verbose:
-                print(f"  program {scene.geo} f64={cfg.double_support} fp={fp or 'strict'}: "
-                      f"{'cached' if hit else 'compiled'} in {time.time() - t:.1f}s")
+            jobs.append((scene, cfg, fp, None))
+    return jobs
 
 
 def precompile_batch_programs(verbose: bool = False) -> None:
@@ -132,7 +133,7 @@ def _suite_jobs():
         spec.loader.exec_module(mod)
         return mod
 
-    jobs = []  # (scene, config, fp options, root source or None)
+    jobs = _standard_jobs()  # (scene, config, fp options, root source or None[, root name])
     par = load("test_gpu_parity")
     if par is not None:
         for make in par.WORKLOADS.values():
@@ -206,12 +207,13 @@ def _suite_shard(k: int, n: int) -> int:
 
 
 def precompile_suite_programs(verbose: bool = False, workers: int = 8) -> None:
-    """The programs the GPU tests and bench.py's legs load, so that a GPU run
-    compiles nothing: the render tuner's candidates for the cell frames and
-    the SDF scene, the plain and FP64 programs of each leg, the wavefront
-    forms, and every scene x option pair of the parity tables. Cached
-    programs cost only their hash; missing ones compile in `workers`
-    processes (each writes the cache through its own temporary files)."""
+    """The standard programs and those the GPU tests and bench.py's legs
+    load, so that a GPU run compiles nothing: the render tuner's candidates
+    for the cell frames and the SDF scene, the plain and FP64 programs of
+    each leg, the wavefront forms, and every scene x option pair of the
+    parity tables. Cached programs cost only their hash; missing ones
+    compile in `workers` processes (each writes the cache through its own
+    temporary files), the longest jobs spread over all of them."""
     t0 = time.time()
     n = len(_suite_jobs())
     procs = [subprocess.Popen([sys.executable, "-c",
@@ -254,7 +256,6 @@ def build_all(verbose: bool = True) -> None:
     if verbose:
         print(f"built {LIB} in {time.time() - t:.1f}s")
     start = time.time() - 1.0
-    precompile_programs(verbose)
     precompile_batch_programs(verbose)
     precompile_suite_programs(verbose)
     prune_cache(start, verbose)

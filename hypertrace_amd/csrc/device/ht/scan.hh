@@ -365,27 +365,31 @@ struct CullRayEu {
 // centre at p.xy + gam u (gam = p.z d.z / |d.xy|), radius p.z / |d.xy|; the
 // part ahead of p is the part with horizontal coordinate >= 0 along u.
 struct CullRayHy {
-    double px, py, pz, dx, dy, dz, ux, uy, gam, rg, rg2, four_rg2;
+    double px, py, pz, dx, dy, dz, dn2, ux, uy, gam, rg2, four_rg2;
     bool valid;
+    // One square root and one division: the cull state only has to be
+    // accurate far inside the bounds' margins, and it is rebuilt for every
+    // (group, ray) pair of the grouped scan.
     static HD CullRayHy make(const Ray<Hy> &r) {
+        HT_CULL_FP
         CullRayHy c;
         double dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
         double pz = r.start.z;
-        double dxy = sqrt_(dx * dx + dy * dy);
-        double dn = sqrt_(dx * dx + dy * dy + dz * dz);
-        c.valid = dxy > 1e-4 * dn && pz > 0.0;
-        if (!c.valid) dxy = 1.0;
+        double dxy2 = dx * dx + dy * dy;
+        double dn2 = dxy2 + dz * dz;
+        c.valid = dxy2 > 1e-8 * dn2 && pz > 0.0;  // |d.xy| > 1e-4 |d|
+        double inv = 1.0 / sqrt_(c.valid ? dxy2 : 1.0);
         c.px = r.start.x;
         c.py = r.start.y;
         c.pz = pz;
-        c.dx = dx / dn;
-        c.dy = dy / dn;
-        c.dz = dz / dn;
-        c.ux = dx / dxy;
-        c.uy = dy / dxy;
-        c.gam = pz * dz / dxy;
-        c.rg = pz * dn / dxy;
-        c.rg2 = c.rg * c.rg;
+        c.dx = dx;
+        c.dy = dy;
+        c.dz = dz;
+        c.dn2 = dn2;
+        c.ux = dx * inv;
+        c.uy = dy * inv;
+        c.gam = pz * dz * inv;
+        c.rg2 = pz * pz * dn2 * (inv * inv);  // (p.z |d| / |d.xy|)^2
         c.four_rg2 = 4.0 * c.rg2;
         return c;
     }
@@ -437,10 +441,11 @@ HD bool CullRayHy::toward_tangent_point(double tx, double ty) const {
     double vx = tx - px, vy = ty - py;
     double v2 = vx * vx + vy * vy, h = pz;
     // tangent at p of the circle through p and T orthogonal to the boundary,
-    // scaled by |v|: (h v, (|v|^2 - h^2) / 2)
+    // scaled by |v|: (h v, (|v|^2 - h^2) / 2); d is not normalised, so both
+    // sides carry |d|^2
     double tz = 0.5 * (v2 - h * h);
     double n = h * (dx * vx + dy * vy) + dz * tz;
-    double den2 = h * h * v2 + tz * tz;
+    double den2 = (h * h * v2 + tz * tz) * dn2;
     return den2 - n * n < 1e-2 * den2;
 }
 

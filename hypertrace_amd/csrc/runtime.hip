@@ -556,6 +556,25 @@ extern "C" int ht_prog_precompile(const char *root_name, const char *root_src, i
                            fp_options, &code);
 }
 
+extern "C" int ht_prog_kernel_info(ht_prog *prog, const char *kernel, int *vgprs, int *scratch_bytes,
+                                   int *lds_bytes) {
+    if (!prog || !kernel) return fail(HT_ERR_OTHER, "ht_prog_kernel_info: invalid argument");
+    HIP_TRY(hipSetDevice(prog->ctx->device));
+    hipFunction_t f = nullptr;
+    if (hipModuleGetFunction(&f, prog->module, kernel) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(HT_ERR_OTHER, std::string("ht_prog_kernel_info: the program has no kernel ") + kernel);
+    }
+    int v = 0, scratch = 0, lds = 0;
+    HIP_TRY(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, f));
+    HIP_TRY(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f));
+    HIP_TRY(hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, f));
+    if (vgprs) *vgprs = v;
+    if (scratch_bytes) *scratch_bytes = scratch;
+    if (lds_bytes) *lds_bytes = lds;
+    return HT_OK;
+}
+
 extern "C" void ht_prog_destroy(ht_prog *prog) {
     if (!prog) return;
     (void)hipSetDevice(prog->ctx->device);

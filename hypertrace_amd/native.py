@@ -24,6 +24,7 @@ SIGNATURES = {
                                  c_int, c_char_p, POINTER(c_void_p)]),
     "ht_prog_destroy": (None, [c_void_p]),
     "ht_prog_cache_hit": (c_int, []),
+    "ht_prog_kernel_info": (c_int, [c_void_p, c_char_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
     "ht_prog_precompile": (c_int, [c_char_p, c_char_p, c_int, POINTER(c_char_p), POINTER(c_char_p), c_int, c_int,
                                    c_char_p]),
     "ht_buf_alloc": (c_int, [c_void_p, c_size_t, POINTER(c_void_p)]),

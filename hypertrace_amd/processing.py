@@ -219,6 +219,14 @@ class Program:
         self.handle = h
         self.scene_type = scene_type_key(scene, ctx.config) if scene is not None else None
 
+    def kernel_info(self, kernel: str = "render_passes") -> dict:
+        """Vector registers and scratch bytes per lane (register spills) and
+        static LDS bytes per block of one of the program's kernels."""
+        v, sc, lds = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        native.check(native.lib().ht_prog_kernel_info(self.handle, kernel.encode(), ctypes.byref(v),
+                                                      ctypes.byref(sc), ctypes.byref(lds)))
+        return {"vgprs": v.value, "scratch": sc.value, "lds": lds.value}
+
     def close(self) -> None:
         if self.handle:
             native.lib().ht_prog_destroy(self.handle)

@@ -47,7 +47,8 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 0, False, True), (4, 5, True, 0, False, True), (4, 6, True, 4, False, True),
     (4, 4, True, 0, True, True), (4, 4, True, 4, True, True),
     (4, 4, True, 4, False, True, True), (4, 4, True, 8, False, True, True), (4, 6, True, 4, False, True, True),
-    (4, 4, True, 4, True, True, True), (4, 6, True, 4, True, True, True))
+    (4, 4, True, 4, True, True, True), (4, 6, True, 4, True, True, True),
+    (4, 3, True, 4, False, True, True), (4, 3, True, 4, True, True, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
@@ -81,14 +82,36 @@ def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
     return options(*_knobs(candidate)[:2], base, *_knobs(candidate)[2:])
 
 
+# A candidate that spills registers is picked only when it beats the best
+# spill-free candidate by more than this fraction of that candidate's time.
+SPILL_TOLERANCE = 0.015
+
+
+def pick(table: Sequence[Dict], tolerance: float = SPILL_TOLERANCE) -> Dict:
+    """The fastest candidate whose render_passes kernel spills no registers
+    to scratch (scratch bytes 0), unless a spilling one is faster by more
+    than `tolerance` (then the spills are out of its hot loop, or cheaper
+    than the occupancy a spill-free register budget costs); the fastest of
+    all when every candidate spills."""
+    fastest = min(table, key=lambda t: t["ms"])
+    clean = [t for t in table if not t.get("scratch")]
+    if clean:
+        best_clean = min(clean, key=lambda t: t["ms"])
+        if fastest["ms"] * (1.0 + tolerance) >= best_clean["ms"]:
+            return best_clean
+    return fastest
+
+
 def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mode: int = 1, seed: int = 0,
                 candidates: Sequence[Sequence] = CANDIDATES, reps: int = 3, base: str = STRICT_FP,
-                cache_path: Optional[str] = None) -> Dict:
+                cache_path: Optional[str] = None, spill_tolerance: float = SPILL_TOLERANCE) -> Dict:
     """Times `spp` passes of render_passes (HIP events, median of `reps`) for
-    each (unroll, waves[, scalar bounds[, groups[, pixel queue[, wave priority[, group queue]]]]]) candidate;
-    returns {"best": {...},
-    "table": [...]} with the program options of the best."""
-    key = f"gfx950:{scene.cpp()}:{width}x{height}:{spp}:{base}"
+    each (unroll, waves[, scalar bounds[, groups[, pixel queue[, wave priority[, group queue]]]]]) candidate,
+    with the kernel's registers and scratch bytes per lane; returns {"best":
+    {...}, "table": [...], "fastest_any": {...}}: the best is the fastest
+    candidate that spills no registers unless a spilling one is faster by
+    more than `spill_tolerance` (see `pick`)."""
+    key = f"gfx950:{scene.cpp()}:{width}x{height}:{spp}:{base}:spill{spill_tolerance}"
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as f:
             known = json.load(f)
@@ -108,11 +131,14 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             ctx.timer_begin()
             r.render_passes(buf, c, spp)
             times.append(ctx.timer_end())
+        res = r.program.kernel_info("render_passes")
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
                       "pixel_queue": queue, "wave_prio": prio, "group_queue": gq,
-                      "ms": sorted(times)[len(times) // 2], "options": opts})
-    best = min(table, key=lambda t: t["ms"])
-    result = {"best": best, "table": table}
+                      "ms": sorted(times)[len(times) // 2], "options": opts,
+                      "vgprs": res["vgprs"], "scratch": res["scratch"]})
+    best = pick(table, spill_tolerance)
+    result = {"best": best, "table": table, "fastest_any": min(table, key=lambda t: t["ms"]),
+              "spill_tolerance": spill_tolerance}
     if cache_path:
         known = {}
         if os.path.exists(cache_path):

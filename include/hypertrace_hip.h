@@ -84,6 +84,11 @@ int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *root_src, i
 void ht_prog_destroy(ht_prog *prog);
 /* 1 if the last build on this thread was served from the cache. */
 int ht_prog_cache_hit(void);
+/* Resources of one kernel of a program (e.g. "render_passes"): vector
+ * registers per lane, scratch bytes per lane (register spills) and static
+ * LDS bytes per block; NULL outputs are skipped. No reference counterpart:
+ * the render tuner reads it to keep programs that spill out of its picks. */
+int ht_prog_kernel_info(ht_prog *prog, const char *kernel, int *vgprs, int *scratch_bytes, int *lds_bytes);
 /* Compile only, into the code-object cache; needs no GPU. No reference
  * counterpart: lets a build step prepare programs ahead of time. */
 int ht_prog_precompile(const char *root_name, const char *root_src, int n_headers,

@@ -1008,27 +1008,28 @@ struct CullRayEu {
 // the behind-the-start case (the horosphere test accepts crossings behind
 // the start for near-vertical local rays, shape/hy/horosphere.cc:16-24).
 struct CullRayHy {
-    CntD px, py, pz, dx, dy, dz, ux, uy, gam, rg, rg2, four_rg2;
+    CntD px, py, pz, dx, dy, dz, dn2, ux, uy, gam, rg2, four_rg2;
     bool valid = false;
+    // one square root and one division, as the kernel builds it
     template <class R> static CullRayHy make(const V4<R> &p, const V4<R> &d) {
         CullRayHy c;
         CntD x = tofloat(d.x), y = tofloat(d.y), z = tofloat(d.z);
         CntD pz = tofloat(p.z);
-        CntD dxy = sqrtd(x * x + y * y);
-        CntD dn = sqrtd(x * x + y * y + z * z);
-        c.valid = dxy.v > (CntD(1e-4) * dn).v && pz.v > 0.0;
-        if (!c.valid) dxy = 1.0;
+        CntD dxy2 = x * x + y * y;
+        CntD dn2 = dxy2 + z * z;
+        c.valid = dxy2.v > (CntD(1e-8) * dn2).v && pz.v > 0.0;
+        CntD inv = CntD(1.0) / sqrtd(c.valid ? dxy2 : CntD(1.0));
         c.px = tofloat(p.x);
         c.py = tofloat(p.y);
         c.pz = pz;
-        c.dx = x / dn;
-        c.dy = y / dn;
-        c.dz = z / dn;
-        c.ux = x / dxy;
-        c.uy = y / dxy;
-        c.gam = pz * z / dxy;
-        c.rg = pz * dn / dxy;
-        c.rg2 = c.rg * c.rg;
+        c.dx = x;
+        c.dy = y;
+        c.dz = z;
+        c.dn2 = dn2;
+        c.ux = x * inv;
+        c.uy = y * inv;
+        c.gam = pz * z * inv;
+        c.rg2 = pz * pz * dn2 * (inv * inv);
         c.four_rg2 = CntD(4.0) * c.rg2;
         return c;
     }
@@ -1037,7 +1038,7 @@ struct CullRayHy {
         CntD v2 = vx * vx + vy * vy, h = pz;
         CntD tz = CntD(0.5) * (v2 - h * h);
         CntD n = h * (dx * vx + dy * vy) + dz * tz;
-        CntD den2 = h * h * v2 + tz * tz;
+        CntD den2 = (h * h * v2 + tz * tz) * dn2;
         return (den2 - n * n).v < (CntD(1e-2) * den2).v;
     }
     // branch-free, as the kernel evaluates it: every term is computed; the

@@ -82,6 +82,13 @@ extern "C" {
     ) -> c_int;
     pub fn ht_prog_destroy(prog: *mut ht_prog);
     pub fn ht_prog_cache_hit() -> c_int;
+    pub fn ht_prog_kernel_info(
+        prog: *mut ht_prog,
+        kernel: *const c_char,
+        vgprs: *mut c_int,
+        scratch_bytes: *mut c_int,
+        lds_bytes: *mut c_int,
+    ) -> c_int;
     pub fn ht_prog_precompile(
         root_name: *const c_char,
         root_src: *const c_char,

@@ -97,3 +97,18 @@ def test_summary_of_failed_legs_stays_small():
     s = bench.summary(out)
     assert len(json.dumps(s).encode()) <= 2048
     assert s["legs"]["sdf_1080p"]["error"].startswith("RuntimeError")
+
+
+def test_bias_variance_fit_recovers_known_terms():
+    import numpy as np
+
+    bench = _bench()
+    b, v = 0.006, 0.02
+    curve = [(n, float(np.sqrt(b * b + v / n))) for n in (8, 16, 32, 64, 128, 256)]
+    fit = bench.bias_variance(curve)
+    assert abs(fit["floor_rmse"] - b) < 1e-9 and abs(fit["var_per_spp"] - v) < 1e-9
+    assert abs(bench.spp_for(fit, 0.01) - v / (1e-4 - b * b)) < 1e-6
+    # a floor above the target: no number of samples reaches it
+    assert bench.spp_for(bench.bias_variance([(n, float(np.sqrt(0.012 ** 2 + v / n))) for n in (8, 64, 512)]),
+                         0.01) is None
+    assert bench.bias_variance([(4, 0.1)]) is None

@@ -91,3 +91,18 @@ def test_bench_fixed_knobs_and_gpu_count_guard():
     r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 2 and "one GPU only" in r.stderr
+
+
+def test_pick_prefers_spill_free_programs():
+    from hypertrace_amd.tuning import pick
+
+    # a spill-free candidate within the tolerance wins over a faster spilling one
+    table = [{"ms": 10.0, "scratch": 24}, {"ms": 10.1, "scratch": 0}, {"ms": 12.0, "scratch": 0}]
+    assert pick(table) is table[1]
+    # beyond the tolerance the spilling one wins
+    table = [{"ms": 10.0, "scratch": 24}, {"ms": 11.0, "scratch": 0}]
+    assert pick(table) is table[0]
+    assert pick(table, tolerance=0.2) is table[1]
+    # every candidate spills: the fastest
+    spilling = [{"ms": 10.0, "scratch": 24}, {"ms": 9.0, "scratch": 100}]
+    assert pick(spilling) is spilling[1]
