@@ -61,11 +61,15 @@ def count_mode(knobs):
     """The oracle's counting mode for the scan a tuner setting runs: 1 = the
     flat culled scan; g = two-level over groups of g (the group tests, then
     the leaf tests of the groups each ray passes); 1000 + g = the same with
-    the group queue (the pair's cull state rebuilt per passing group)."""
+    the group queue (the pair's cull state rebuilt per passing group); 2000
+    + g = the group queue with the hit order (crossing intervals of the
+    passing pairs, exact tests only of the pairs the interval filter keeps)."""
     g = int(knobs.get("groups") or 0)
     if not g or not knobs.get("scalar_bounds"):
         return 1  # groups over LDS bounds build their tree per block; counted as the flat scan's tests
-    return (1000 if knobs.get("group_queue") else 0) + g
+    if knobs.get("group_queue"):
+        return (2000 if knobs.get("hit_order") else 1000) + g
+    return g
 
 
 def count_flops_per_path(scene_spec, w, h, seeds, stride=16, mode=1):
@@ -535,7 +539,7 @@ def spp_for(fit, target):
 
 
 def fixed_knobs(text, base=None):
-    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq]" (no timing)."""
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho]" (no timing)."""
     from hypertrace_amd.processing import STRICT_FP
     from hypertrace_amd.tuning import options
     knobs = text.split(",")
@@ -546,10 +550,11 @@ def fixed_knobs(text, base=None):
     queue = "q" in flags
     prio = "p" in flags
     gq = "gq" in flags
+    ho = "ho" in flags
     return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                     "pixel_queue": queue, "wave_prio": prio, "group_queue": gq,
+                     "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho,
                      "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups,
-                                        pixel_queue=queue, wave_prio=prio, group_queue=gq)},
+                                        pixel_queue=queue, wave_prio=prio, group_queue=gq, hit_order=ho)},
             "table": [], "fixed": True}
 
 
@@ -771,10 +776,10 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
-    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq]",
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho]",
                     help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
                          "instead of tuning (keeps a profiled trace to one program)")
-    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq]",
+    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho]",
                     help="the same for the SDF leg's program")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "

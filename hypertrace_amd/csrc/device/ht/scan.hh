@@ -61,7 +61,16 @@ static constexpr int WAVE = 64;
 #define HT_RING_STEP HT_CULL_UNROLL
 #endif
 static constexpr uint RING_STEP = HT_RING_STEP;
+#ifdef HT_HIT_ORDER
+// With HT_HIT_ORDER the exact tests wait in the ring until it may not take
+// another level-2 batch (one leaf per group leaf and lane), so that most
+// scans filter all their candidates against the ray's final certain hit.
+static constexpr uint QUEUE_RING = 512;
+static_assert(RING_STEP <= 4, "the hit-order ring holds one level-2 batch of groups of up to 4 leaves");
+static constexpr uint HIT_ORDER_DRAIN = QUEUE_RING - WAVE * RING_STEP;
+#else
 static constexpr uint QUEUE_RING = WAVE * (RING_STEP + 1);
+#endif
 HD uint ring_wrap(uint i) { return i >= QUEUE_RING ? i - QUEUE_RING : i; }
 // Program option HT_GROUP_QUEUE (with HT_SCALAR_BOUNDS and HT_CULL_GROUPS):
 // the two-level scan compacts at both levels. The group spheres are tested
@@ -79,6 +88,42 @@ HD uint gring_wrap(uint i) { return i >= GROUP_RING ? i - GROUP_RING : i; }
 #endif
 #if defined(HT_GROUP_QUEUE) && defined(HT_BATCH_BLOCK)
 #error "HT_GROUP_QUEUE is an option of scene programs"
+#endif
+// Program option HT_HIT_ORDER (with HT_GROUP_QUEUE): before the exact tests,
+// each passing (element, ray) pair gets the interval of the ray's
+// hyperbolic distance at which its geodesic crosses the element's outer
+// sphere, and a "certain hit" bound where the geodesic crosses the inner
+// sphere (the element's surface lies between the two spheres, so the
+// exact test must hit it no later). A pair whose geodesic never crosses
+// the outer sphere ahead of the start is dropped; a pair whose entry lies
+// at least HIT_ORDER_MARGIN beyond the ray's nearest certain hit is skipped.
+// Neither can be the reference loop's nearest hit: the exact tests' FP32
+// distances of these scenes stay within 1e-3 of the geometric ones, far
+// inside the margin. The winner, its distance and its hit state are those
+// of the plain scan (the exact test decides every hit that is kept).
+#if defined(HT_HIT_ORDER) && !defined(HT_GROUP_QUEUE)
+#error "HT_HIT_ORDER needs HT_GROUP_QUEUE"
+#endif
+#ifdef HT_HIT_ORDER
+static_assert(HT_BOUNDS_CAP <= 1024, "hit-order ring entries hold a 10-bit element index");
+// exp(0.02): a skipped pair's entry is at least 0.02 farther (hyperbolic
+// distance) than the nearest certain hit; rounded up for the float product.
+#define HIT_ORDER_MARGIN 1.0203f
+#endif
+// Pair-ring entry: (element << 6) | ray lane; with HT_HIT_ORDER the upper
+// 16 bits hold the pair's lb (the upper half of its float bits).
+#ifdef HT_HIT_ORDER
+#define HIT_LB_ONE 0x3F80u     // 1.0f: no interval
+#define HIT_LB_PENDING 0u      // the interval is not computed yet
+#define HIT_LB_FIELD(lb16) ((uint)(lb16) << 16)
+#define HIT_ELEM(e) (((e) >> 6) & 0x3FFu)
+#define HIT_ORDER_LEAF(slot, el) (slot)
+#else
+#define HIT_LB_ONE 0u
+#define HIT_LB_PENDING 0u
+#define HIT_LB_FIELD(lb16) 0u
+#define HIT_ELEM(e) ((e) >> 6)
+#define HIT_ORDER_LEAF(slot, el) (el)
 #endif
 #ifndef HT_MAX_WAVES
 #define HT_MAX_WAVES 4  // blocks of up to 256 threads
@@ -339,8 +384,27 @@ template <class T, class M> HD Bound element_bound(const u8 *elem) {
 }
 
 // ------------------------------------------------------------ ray cull tests
+#ifdef HT_HIT_ORDER
+// Crossing interval of a (ray, element) pair; see CullRayHy::interval.
+struct HitIv {
+    float lb, ub;
+    bool drop, certain;
+};
+HD float float_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = __uint_as_float(__float_as_uint(f) - 1u);  // x >= 1 here
+    return f;
+}
+HD float float_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = __uint_as_float(__float_as_uint(f) + 1u);
+    return f;
+}
+#endif
+
 // Euclidean ray p + t d, t >= 0.
 struct CullRayEu {
+    static constexpr bool HYPERBOLIC = false;
     double px, py, pz, dx, dy, dz;
     bool valid;
     static HD CullRayEu make(const Ray<Eu> &r) {
@@ -349,7 +413,7 @@ struct CullRayEu {
     }
     // Branch-free: every term is evaluated, so several elements' tests can
     // be interleaved; the decision is the same boolean function.
-    template <class B> HD bool candidate(const B &b) const {
+    template <bool NODE = false, class B> HD bool candidate(const B &b) const {
         HT_CULL_FP
         double sx = b_cx(b) - px, sy = b_cy(b) - py, sz = b_cz(b) - pz;
         double r = b_r(b);
@@ -360,25 +424,35 @@ struct CullRayEu {
     }
 };
 
-// Hyperbolic geodesic from p with unit tangent d: the circle centred on the
+// Hyperbolic geodesic from p with tangent d: the circle centred on the
 // boundary plane in the vertical plane through p along u = d.xy / |d.xy|,
-// centre at p.xy + gam u (gam = p.z d.z / |d.xy|), radius p.z / |d.xy|; the
-// part ahead of p is the part with horizontal coordinate >= 0 along u.
+// centre at p.xy + gam u (gam = p.z d.z / |d.xy|), radius
+// rg = p.z |d| / |d.xy|; the part ahead of p is the part with horizontal
+// coordinate >= 0 along u. A sphere (centre c, radius r) is culled when the
+// plane misses it, its cut (centre at sig along u and height c.z, radius
+// rp) lies wholly behind p, the two circles are disjoint, or one lies
+// strictly inside the other. The tests are written multiplied through by
+// m = |d.xy|^2 (and m^2), with S = sig |d.xy| = (c - p).xy . d.xy,
+// D = delta |d.xy| = (c - p).xy x d.xy, RP = rp^2 m = r^2 m - D^2 and
+// O = (d^2 - rg^2 - rp^2) m = S^2 - 2 p.z d.z S + (c.z^2 - p.z^2) m - RP
+// (d: the distance between the circle centres), so the cull state needs no
+// square root and no division: disjoint <=> O > 0 and O^2 > 4 p.z^2 |d|^2 RP;
+// the cut inside the geodesic's circle <=> p.z^2 |d|^2 > RP, O < 0 and the
+// same square test; the geodesic's circle inside the inner sphere's cut
+// <=> RPI > p.z^2 |d|^2, I = (ri^2 - r^2) m - O > 0 and
+// I^2 > 4 p.z^2 |d|^2 RPI (RPI = ri^2 m - D^2).
 struct CullRayHy {
-    double px, py, pz, dx, dy, dz, dn2, ux, uy, gam, rg2, four_rg2;
+    static constexpr bool HYPERBOLIC = true;
+    double px, py, pz, dx, dy, dz, dn2, m, pz2, pz2n, four_pz2n, two_pzdz;
     bool valid;
-    // One square root and one division: the cull state only has to be
-    // accurate far inside the bounds' margins, and it is rebuilt for every
-    // (group, ray) pair of the grouped scan.
     static HD CullRayHy make(const Ray<Hy> &r) {
         HT_CULL_FP
         CullRayHy c;
         double dx = r.direction.x, dy = r.direction.y, dz = r.direction.z;
         double pz = r.start.z;
-        double dxy2 = dx * dx + dy * dy;
-        double dn2 = dxy2 + dz * dz;
-        c.valid = dxy2 > 1e-8 * dn2 && pz > 0.0;  // |d.xy| > 1e-4 |d|
-        double inv = 1.0 / sqrt_(c.valid ? dxy2 : 1.0);
+        double m = dx * dx + dy * dy;
+        double dn2 = m + dz * dz;
+        c.valid = m > 1e-8 * dn2 && pz > 0.0;  // |d.xy| > 1e-4 |d|
         c.px = r.start.x;
         c.py = r.start.y;
         c.pz = pz;
@@ -386,49 +460,55 @@ struct CullRayHy {
         c.dy = dy;
         c.dz = dz;
         c.dn2 = dn2;
-        c.ux = dx * inv;
-        c.uy = dy * inv;
-        c.gam = pz * dz * inv;
-        c.rg2 = pz * pz * dn2 * (inv * inv);  // (p.z |d| / |d.xy|)^2
-        c.four_rg2 = 4.0 * c.rg2;
+        c.m = m;
+        c.pz2 = pz * pz;
+        c.pz2n = c.pz2 * dn2;
+        c.four_pz2n = 4.0 * c.pz2n;
+        c.two_pzdz = 2.0 * pz * dz;
         return c;
     }
     HD bool toward_tangent_point(double tx, double ty) const;
     // Branch-free (the same boolean function as the sequence of tests it
-    // describes), so several elements' tests can be interleaved.
-    template <class B> HD bool candidate(const B &b) const {
+    // describes), so several elements' tests can be interleaved. NODE: a
+    // tree node's sphere (inner radius 0, so the last test never culls; a
+    // node over a horoball keeps the "behind" case).
+    template <bool NODE = false, class B> HD bool candidate(const B &b) const {
         HT_CULL_FP
         double rx = b_cx(b) - px, ry = b_cy(b) - py;
-        double delta = -rx * uy + ry * ux;  // distance from the vertical plane
-        double d2p = delta * delta;
-        double rp2 = b_r2(b) - d2p;  // squared radius of the plane's cut of the outer sphere
-        double sig = rx * ux + ry * uy;  // along the plane, from p
-        double ex = sig - gam;
-        double d2 = ex * ex + b_cz2(b);  // squared distance between the circle centres
+        double S = rx * dx + ry * dy;   // along the plane, from p (x |d.xy|)
+        double D = -rx * dy + ry * dx;  // distance from the vertical plane (x |d.xy|)
+        double d2 = D * D;
+        double rp = b_r2(b) * m - d2;   // squared radius of the plane's cut of the outer sphere (x m)
+        double s2 = S * S;
         // 1. the plane misses the outer sphere
-        bool cull = rp2 < 0.0;
+        bool cull = rp < 0.0;
         // 2. the cut circle lies wholly behind p (horizontal coordinate < 0);
         //    not applied to a horoball when the ray is near the geodesic
         //    through its tangent point T: there the element's local ray is
         //    near vertical, and the horosphere test (shape/hy/horosphere.cc:16-24)
         //    applies its EPS window before dividing by |d.xy|^2, so it accepts
         //    crossings behind the start.
-        bool behind = sig < 0.0 && sig * sig > rp2;
+        bool behind = S < 0.0 && s2 > rp;
         // horo = 2 marks a tree node whose subtree holds a horoball: no "behind" cull
         float horo = b_horo(b);
-        if (behind && horo > 0) behind = horo < 1.5f && !toward_tangent_point(b_cx(b), b_cy(b));
+        if constexpr (NODE) {
+            behind = behind && !(horo > 0);
+        } else {
+            if (behind && horo > 0) behind = horo < 1.5f && !toward_tangent_point(b_cx(b), b_cy(b));
+        }
         cull = cull || behind;
-        // 3. disjoint circles: d2 > (rg + rp)^2  <=>  d2 - rg^2 - rp^2 > 2 rg rp
-        double o = d2 - rg2 - rp2;
-        double f_rp2 = four_rg2 * rp2;
-        cull = cull || (o > 0.0 && o * o > f_rp2);
-        // 4. outer cut circle strictly inside the geodesic circle: rp < rg, d2 < (rg - rp)^2
-        double in1 = rg2 + rp2 - d2;
-        cull = cull || (rg2 > rp2 && in1 > 0.0 && in1 * in1 > f_rp2);
-        // 5. geodesic circle strictly inside the inner sphere's cut: rg < rpi, d2 < (rpi - rg)^2
-        double rpi2 = b_ri2(b) - d2p;
-        double in2 = rg2 + rpi2 - d2;
-        cull = cull || (rpi2 > rg2 && in2 > 0.0 && in2 * in2 > four_rg2 * rpi2);
+        // 3. disjoint circles; 4. the cut circle strictly inside the geodesic circle
+        double o = s2 - two_pzdz * S + (b_cz2(b) - pz2) * m - rp;
+        double o2 = o * o, f = four_pz2n * rp;
+        cull = cull || (o > 0.0 && o2 > f);
+        cull = cull || (pz2n > rp && o < 0.0 && o2 > f);
+        // 5. the geodesic circle strictly inside the inner sphere's cut
+        if constexpr (!NODE) {
+            double ri2 = b_ri2(b);
+            double rpi = ri2 * m - d2;
+            double in2 = (ri2 - b_r2(b)) * m - o;
+            cull = cull || (rpi > pz2n && in2 > 0.0 && in2 * in2 > four_pz2n * rpi);
+        }
         return b_r(b) < 0 || !valid || !cull;
     }
 };
@@ -449,6 +529,139 @@ HD bool CullRayHy::toward_tangent_point(double tx, double ty) const {
     return den2 - n * n < 1e-2 * den2;
 }
 
+#ifdef HT_HIT_ORDER
+// Crossing interval of a geodesic ray with an element's bound spheres.
+//
+// Parametrise the ray's semicircle by the angle theta seen from its centre,
+// measured from the end E the ray moves to; the start is at theta_p and
+// along the ray t = tan(theta / 2) falls from tp = tan(theta_p / 2) to 0.
+// The hyperbolic distance from the start is ln(tp / t) (arc length along a
+// geodesic semicircle), so with w = tp / t = e^D the crossings of a sphere
+// with the semicircle are the roots of gam w^2 - 2 bet w + alp = 0, where
+// in units of p.z (S, Z: the sphere centre's coordinates along the ray's
+// horizontal direction and in height, P: the radius of the sphere's cut by
+// the ray's vertical plane):
+//   alp = (1 + S tp)^2 + (Z tp)^2 - P^2 tp^2   (the power of the far end W)
+//   gam = (tp - S)^2 + Z^2 - P^2               (the power of E)
+//   bet = (tp^2 + 1) Z
+// and the start (w = 1) is inside the sphere iff S^2 + (1 - Z)^2 < P^2.
+// With q = bet + sqrt(bet^2 - alp gam) (bet >= 0) the roots are q / gam and
+// alp / q; the first crossing ahead of the start is alp / q.
+//   drop:    the start is outside the outer sphere and no crossing of it
+//            lies ahead (no root, or both roots below 1);
+//   lb:      e^D of the first crossing of the outer sphere ahead (1 when
+//            the start is inside it), rounded down;
+//   certain: the start is outside the outer sphere and the geodesic
+//            crosses the inner sphere ahead, at ub = e^D rounded up: the
+//            element's surface lies between the spheres, so its first
+//            crossing ahead is no farther than ub.
+// Exempt pairs (the element the ray leaves, whose test applies the repeat
+// rule, and a horoball whose test may accept crossings behind the start)
+// get lb = 1 and no certain bound.
+// Reciprocal and square root in FP64 from the hardware approximations and
+// one Newton step (relative error far below the interval's 1e-6 slack).
+HD double rcp_nr(double x) {
+    HT_CULL_FP
+    double r = __builtin_amdgcn_rcp(x);
+    return r + r * (1.0 - x * r);
+}
+HD double sqrt_nr(double x) {
+    HT_CULL_FP
+    if (!(x > 0.0)) return 0.0;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * (1.5 - 0.5 * x * y * y);
+    return x * y;
+}
+// Per-ray constants of hit_interval, one float4 per ray: the horizontal unit
+// direction (ux, uy), tp and 1 / p.z (tp = 0: no interval for this ray, as
+// where the cull state is not valid; straight rays have none).
+HD float4 hit_interval_consts(const Ray<Eu> &) { return make_float4(0.f, 0.f, 0.f, 0.f); }
+HD float4 hit_interval_consts(const Ray<Hy> &r) {
+    HT_CULL_FP
+    double dx = r.direction.x, dy = r.direction.y, dz = r.direction.z, pz = r.start.z;
+    double m = dx * dx + dy * dy, n = m + dz * dz;
+    if (!(m > 1e-8 * n && pz > 0.0)) return make_float4(0.f, 0.f, 0.f, 0.f);
+    double sm = sqrt_(m), dn = sqrt_(n), inv = 1.0 / sm;
+    // (|d| + d.z) / |d.xy|, written without cancellation for d.z < 0
+    double tp = dz >= 0.0 ? (dn + dz) * inv : sm / (dn - dz);
+    return make_float4((float)(dx * inv), (float)(dy * inv), (float)tp, (float)(1.0 / pz));
+}
+// True if the direction d at p is within ~6 degrees of the tangent (either
+// sense) of the geodesic from p to the boundary point (tx, ty, 0) (as
+// CullRayHy::toward_tangent_point).
+HD bool toward_tangent(double px, double py, double pz, double dx, double dy, double dz, double tx, double ty) {
+    HT_CULL_FP
+    double vx = tx - px, vy = ty - py;
+    double v2 = vx * vx + vy * vy, h = pz;
+    double tz = 0.5 * (v2 - h * h);
+    double n = h * (dx * vx + dy * vy) + dz * tz;
+    double den2 = (h * h * v2 + tz * tz) * (dx * dx + dy * dy + dz * dz);
+    return den2 - n * n < 1e-2 * den2;
+}
+// The interval of a pair (see above): ray start (px, py, pz) with the
+// constants k of hit_interval_consts; exempt pairs get lb = 1.
+template <class B> HD HitIv hit_interval(const B &b, double px, double py, double pz, float4 k, bool exempt) {
+    HT_CULL_FP
+    HitIv iv = {1.0f, 0.0f, false, false};
+    double tp = k.z, ipz = k.w;
+    if (!(tp > 0.0) || exempt || b_r(b) < 0) return iv;
+    double ux = k.x, uy = k.y;
+    double rx = b_cx(b) - px, ry = b_cy(b) - py;
+    double sig = rx * ux + ry * uy, delta = -rx * uy + ry * ux;
+    double d2p = delta * delta;
+    double S = sig * ipz, Z = b_cz(b) * ipz, ipz2 = ipz * ipz;
+    double po2 = (b_r2(b) - d2p) * ipz2;
+    double zm = 1.0 - Z;
+    (void)pz;
+    if (S * S + zm * zm < po2) return iv;  // the start is inside the outer sphere
+    if (!(po2 > 0.0)) {
+        iv.drop = true;  // the ray's vertical plane misses the sphere
+        return iv;
+    }
+    double tp2 = tp * tp;
+    double u = 1.0 + S * tp, v = tp - S, zt = Z * tp;
+    double a0 = u * u + zt * zt, g0 = v * v + Z * Z, bet = (tp2 + 1.0) * Z;
+    // inner sphere first: a certain pair is never dropped
+    double pi2 = (b_ri2(b) - d2p) * ipz2;
+    if (pi2 > 0.0) {
+        double ai = a0 - pi2 * tp2, gi = g0 - pi2;
+        double di = bet * bet - ai * gi;
+        if (di >= 0.0) {
+            double qi = bet + sqrt_nr(di);
+            if (qi > 0.0 && (gi <= 0.0 || qi >= gi * (1.0 + 1e-6))) {
+                double ub = ai * rcp_nr(qi) * (1.0 + 1e-9);
+                if (ub >= 1.0 && ub < 1e30) {
+                    iv.certain = true;
+                    iv.ub = float_up(ub);
+                }
+            }
+        }
+    }
+    double ao = a0 - po2 * tp2, go = g0 - po2;
+    double disc = bet * bet - ao * go;
+    double q = bet + sqrt_nr(disc);
+    if (!iv.certain && (disc < 0.0 || (go > 0.0 && q < go * (1.0 - 1e-6)))) {
+        iv.drop = true;
+        return iv;
+    }
+    double w = ao * rcp_nr(q);
+    iv.lb = w > 1.0 ? float_down(w * (1.0 - 1e-9)) : 1.0f;
+    return iv;
+}
+// The interval of a leaf's pair with a ray: a horoball whose test may accept
+// crossings behind the start (the ray near the geodesic through its tangent
+// point) is exempt as well; straight rays have no interval.
+template <class B> HD HitIv hit_order_interval(const B &b, const Ray<Hy> &r, float4 k, bool exempt) {
+    if (!exempt && b_horo(b) > 0.f)
+        exempt = toward_tangent(r.start.x, r.start.y, r.start.z, r.direction.x, r.direction.y, r.direction.z, b_cx(b),
+                                b_cy(b));
+    return hit_interval(b, r.start.x, r.start.y, r.start.z, k, exempt);
+}
+template <class B> HD HitIv hit_order_interval(const B &, const Ray<Eu> &, float4, bool) {
+    return {1.0f, 0.0f, false, false};
+}
+#endif
+
 template <class G> struct CullRayOf;
 template <> struct CullRayOf<Eu> { typedef CullRayEu type; };
 template <> struct CullRayOf<Hy> { typedef CullRayHy type; };
@@ -462,6 +675,10 @@ template <class G> struct WaveScratch {
     uint queue[QUEUE_RING];
 #ifdef HT_GROUP_QUEUE
     uint gqueue[GROUP_RING];  // (group << 6) | ray lane
+#endif
+#ifdef HT_HIT_ORDER
+    uint tw[WAVE];       // float bits of the ray's nearest certain hit e^D (+inf: none yet)
+    float4 ivc[WAVE];     // the ray's interval constants (hit_interval_consts)
 #endif
 };
 
@@ -840,10 +1057,15 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         return dist;
     }
 
-    static HD void process(const u8 *self, WaveScratch<G> *ws, uint arank, uint k, uint head) {
+    static HD void process(const u8 *self, WaveScratch<G> *ws, uint arank, uint k, uint head, bool slots = false) {
         if (arank < k) {
             uint e = ws->queue[ring_wrap(head + arank)];
-            uint i = e >> 6, L = e & 63u;
+            uint i = HIT_ELEM(e), L = e & 63u;
+#ifdef HT_HIT_ORDER
+            if (slots) i = __float_as_uint(bounds_lds<OV>()[i].pad1);  // the entry names the tree's leaf
+#else
+            (void)slots;
+#endif
             typename E::Cache cc;
             Ray<G> local;
             real dist = element_test(self, i, ws->ray[L], ws->orig_hasher[L], ws->prev_hash[L], &cc, &local);
@@ -902,7 +1124,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     // push_owned(element, candidate, ray lane) appends to the exact tests'
     // ring, drain() runs its full batches.
     template <class CR, class PushO, class Drain>
-    static HD void walk_groups_queued(const CR &cr, const ConstBoundS *sb, WaveScratch<G> *ws, uint lane,
+    static HD void walk_groups_queued(usize n, const CR &cr, const ConstBoundS *sb, WaveScratch<G> *ws, uint lane,
                                       unsigned long long below, uint nact, uint arank, PushO &push_owned,
                                       Drain &drain) {
         constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP, FIRST = P / GS;
@@ -938,16 +1160,22 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                     el = __float_as_uint(lb.pad1);
                     cand = el != NO_ELEMENT && c2.candidate(lb);
                 }
-                push_owned(el, cand, L);
+                // with HT_HIT_ORDER the entry names the leaf (its interval is
+                // computed when the ring is filtered, one pair per lane)
+                push_owned(HIT_ORDER_LEAF(gk * GS + j, el), cand, L, HIT_LB_PENDING);
             }
             drain();
         };
-        for (uint k0 = 0; k0 < FIRST; k0 += CULL_UNROLL) {
+        // the n elements fill the leaves [0, n) (bounded ones in Morton order,
+        // then the unbounded): the groups after the last one holding an
+        // element are padding only
+        const uint groups = ((uint)n + GS - 1) / GS;
+        for (uint k0 = 0; k0 < groups; k0 += CULL_UNROLL) {
             bool pass[CULL_UNROLL];
 #pragma unroll
             for (int u = 0; u < CULL_UNROLL; ++u) {
                 BoundS q = load_bound(sb + P + k0 + u);
-                pass[u] = q.r != -2.0 && cr.candidate(q);  // -2: a node over padding only
+                pass[u] = q.r != -2.0 && cr.template candidate<true>(q);  // -2: a node over padding only
             }
 #pragma unroll
             for (int u = 0; u < CULL_UNROLL; ++u) gpush(k0 + u, pass[u]);
@@ -961,12 +1189,13 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     // walk_groups over the global array fill_scalar_block writes: every
     // node and leaf bound is one scalar load.
     template <class CR, class Push, class Drain>
-    static HD void walk_groups_scalar(const CR &cr, const ConstBoundS *sb, Push &push, Drain &drain) {
-        constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP, FIRST = P / GS;
-        for (uint k = 0; k < FIRST; ++k) {
+    static HD void walk_groups_scalar(usize n, const CR &cr, const ConstBoundS *sb, Push &push, Drain &drain) {
+        constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP;
+        const uint groups = ((uint)n + GS - 1) / GS;  // the later groups are padding only
+        for (uint k = 0; k < groups; ++k) {
             BoundS q = load_bound(sb + P + k);
             if (q.r == -2.0) continue;  // no elements under this node
-            bool pass = cr.candidate(q);
+            bool pass = cr.template candidate<true>(q);
             if (__ballot(pass) == 0) continue;
             bool c[GS];
             uint el[GS];
@@ -993,7 +1222,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         for (uint k = FIRST; k < 2 * FIRST; ++k) {
             float4 q = t->node[k];
             if (q.w == -2.f) continue;  // no elements under this node
-            bool pass = cr.candidate(MakeBound<Bound>::node(q, t->horo[k] != 0));
+            bool pass = cr.template candidate<true>(MakeBound<Bound>::node(q, t->horo[k] != 0));
             if (__ballot(pass) == 0) continue;
             bool c[GS];
             unsigned short el[GS];
@@ -1029,29 +1258,99 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             ws->orig_hasher[lane] = orig;
             ws->prev_hash[lane] = ctx->prev_hash;
             ws->best[lane] = ~0ull;
+#ifdef HT_HIT_ORDER
+            ws->tw[lane] = 0x7F800000u;  // +inf
+            ws->ivc[lane] = hit_interval_consts(light->ray);
+#endif
             unsigned long long below = lane_mask_below();
             unsigned long long active = __ballot(1);
             uint nact = (uint)__popcll(active);
             uint arank = (uint)__popcll(active & below);
             typename CullRayOf<G>::type cr = CullRayOf<G>::type::make(light->ray);
             wave_sync();
+            bool tree = false;
+            if constexpr (TREE_OK) tree = n >= TREE_MIN_ELEMENTS;
             uint qcount = 0, qhead = 0;
             // Pairs in a ring: push adds element i for the lanes where it is
             // a candidate; drain runs the exact tests of full batches (one
             // pair per active lane) from the head, once per scan step, so the
             // exact test's code appears once in the loop.
-            auto push_owned = [&](usize i, bool cand, uint owner) {
+            // (with HT_HIT_ORDER an entry also holds the upper 16 bits of
+            // the pair's lb as a float: a lower bound)
+            auto push_owned = [&](usize i, bool cand, uint owner, uint lb16) {
                 unsigned long long m = __ballot(cand);
                 if (m == 0) return;
                 if (cand)
-                    ws->queue[ring_wrap(qhead + qcount + (uint)__popcll(m & below))] = ((uint)i << 6) | owner;
+                    ws->queue[ring_wrap(qhead + qcount + (uint)__popcll(m & below))] =
+                        HIT_LB_FIELD(lb16) | ((uint)i << 6) | owner;
                 qcount += (uint)__popcll(m);
             };
-            auto push = [&](usize i, bool cand) { push_owned(i, cand, lane); };
-            auto drain = [&]() {
-                while (qcount >= nact) {
+            auto push = [&](usize i, bool cand) { push_owned(i, cand, lane, HIT_LB_ONE); };
+#ifdef HT_HIT_ORDER
+            // Drops the ring's pairs whose entry lies beyond their ray's
+            // nearest certain hit by the margin (stable in-place compaction,
+            // one pair per active lane per step: a step writes no slot that a
+            // later step reads).
+            auto filter = [&]() {
+                wave_sync();  // the pushes of the last batch
+                if (tree) {
+                    // the intervals of the pairs pushed since the last filter,
+                    // one pair per active lane; a certain pair lowers its
+                    // ray's bound, a dropped one gets lb = +inf
+                    const Bound *leaves = bounds_lds<OV>();
+                    for (uint base = 0; base < qcount; base += nact) {
+                        uint pos = base + arank;
+                        if (pos < qcount) {
+                            uint at = ring_wrap(qhead + pos);
+                            uint e = ws->queue[at];
+                            if ((e >> 16) == HIT_LB_PENDING) {
+                                uint L = e & 63u;
+                                const Bound &b = leaves[HIT_ELEM(e)];
+                                Hasher hh = {ws->orig_hasher[L]};
+                                hash_usize(&hh, __float_as_uint(b.pad1));
+                                bool exempt = hh.state == ws->prev_hash[L];  // the repeat rule decides its test
+                                HitIv iv = hit_order_interval(b, ws->ray[L], ws->ivc[L], exempt);
+                                if (iv.certain) atomicMin(&ws->tw[L], __float_as_uint(iv.ub));
+                                uint lb16 = iv.drop ? 0x7F80u : (__float_as_uint(iv.lb) >> 16);  // truncation rounds down
+                                ws->queue[at] = (e & 0xFFFFu) | (lb16 << 16);
+                            }
+                        }
+                    }
                     wave_sync();
-                    process(self, ws, arank, nact, qhead);
+                }
+                uint out = 0;
+                for (uint base = 0; base < qcount; base += nact) {
+                    uint pos = base + arank;
+                    bool keep = false;
+                    uint e = 0;
+                    if (pos < qcount) {
+                        e = ws->queue[ring_wrap(qhead + pos)];
+                        float lbf = __uint_as_float(e & 0xFFFF0000u);
+                        float t = __uint_as_float(ws->tw[e & 63u]);
+                        keep = !(lbf >= t * HIT_ORDER_MARGIN);
+                    }
+                    unsigned long long km = __ballot(keep);
+                    wave_sync();
+                    if (keep) ws->queue[ring_wrap(qhead + out + (uint)__popcll(km & below))] = e;
+                    out += (uint)__popcll(km);
+                    wave_sync();
+                }
+                qcount = out;
+            };
+#endif
+            auto drain = [&]() {
+#ifdef HT_HIT_ORDER
+                // Wait while the ring can take another level-2 batch; then
+                // filter it and test only as many pairs as make room for one:
+                // the others wait for their ray's final certain bound.
+                if (qcount <= HIT_ORDER_DRAIN) return;
+                filter();
+                while (qcount > HIT_ORDER_DRAIN) {
+#else
+                while (qcount >= nact) {
+#endif
+                    wave_sync();
+                    process(self, ws, arank, nact, qhead, tree);
                     qhead = ring_wrap(qhead + nact);
                     qcount -= nact;
                     wave_sync();
@@ -1061,13 +1360,11 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 push(i, cand);
                 drain();
             };
-            bool tree = false;
-            if constexpr (TREE_OK) tree = n >= TREE_MIN_ELEMENTS;
             if (tree) {
 #if defined(HT_GROUP_QUEUE)
-                if constexpr (TREE_OK) walk_groups_queued(cr, sb, ws, lane, below, nact, arank, push_owned, drain);
+                if constexpr (TREE_OK) walk_groups_queued(n, cr, sb, ws, lane, below, nact, arank, push_owned, drain);
 #elif defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS)
-                if constexpr (TREE_OK) walk_groups_scalar(cr, sb, push, drain);
+                if constexpr (TREE_OK) walk_groups_scalar(n, cr, sb, push, drain);
 #elif defined(HT_CULL_GROUPS)
                 if constexpr (TREE_OK) walk_groups(cr, push, drain);
 #else
@@ -1087,9 +1384,20 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 }
                 for (; e < n; ++e) append(e, cr.candidate(bound(e)));
             }
+#ifdef HT_HIT_ORDER
+            if (qcount > 0) {
+                filter();
+                while (qcount >= nact) {
+                    wave_sync();
+                    process(self, ws, arank, nact, qhead, tree);
+                    qhead = ring_wrap(qhead + nact);
+                    qcount -= nact;
+                }
+            }
+#endif
             if (qcount > 0) {
                 wave_sync();
-                process(self, ws, arank, qcount, qhead);
+                process(self, ws, arank, qcount, qhead, tree);
             }
             wave_sync();
             unsigned long long best = ws->best[lane];

@@ -23,7 +23,12 @@ queue is empty, by the work each wave's current pixels have left);
 HT_GROUP_QUEUE compacts the two-level scan at both levels: the (group,
 ray) pairs that pass a group sphere are queued and tested on full lanes
 against the group's leaves, so each ray pays for the groups it passes
-rather than for the groups any ray of its wave passes. Every
+rather than for the groups any ray of its wave passes; HT_HIT_ORDER (with
+the group queue) gives each passing pair the interval of hyperbolic
+distance at which its geodesic crosses the element's bound spheres, drops
+the pairs whose geodesic never enters the sphere ahead of the start, and
+skips those that lie beyond the ray's nearest certain hit, so most rays
+run one or two exact tests instead of five. Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -48,11 +53,14 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 0, True, True), (4, 4, True, 4, True, True),
     (4, 4, True, 4, False, True, True), (4, 4, True, 8, False, True, True), (4, 6, True, 4, False, True, True),
     (4, 4, True, 4, True, True, True), (4, 6, True, 4, True, True, True),
-    (4, 3, True, 4, False, True, True), (4, 3, True, 4, True, True, True))
+    (4, 3, True, 4, False, True, True), (4, 3, True, 4, True, True, True),
+    (4, 4, True, 4, False, True, True, True), (4, 3, True, 4, False, True, True, True),
+    (4, 4, True, 4, True, True, True, True), (4, 3, True, 4, True, True, True, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
-            pixel_queue: bool = False, wave_prio: bool = False, group_queue: bool = False) -> str:
+            pixel_queue: bool = False, wave_prio: bool = False, group_queue: bool = False,
+            hit_order: bool = False) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
@@ -64,17 +72,20 @@ def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool 
         opts += " -DHT_WAVE_PRIO=1"
     if group_queue:
         opts += " -DHT_GROUP_QUEUE=1"
+    if hit_order:
+        opts += " -DHT_HIT_ORDER=1"
     return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool]:
     unroll, waves = int(candidate[0]), int(candidate[1])
     sb = bool(candidate[2]) if len(candidate) > 2 else False
     groups = int(candidate[3]) if len(candidate) > 3 else 0
     queue = bool(candidate[4]) if len(candidate) > 4 else False
     prio = bool(candidate[5]) if len(candidate) > 5 else False
     gq = bool(candidate[6]) if len(candidate) > 6 else False
-    return unroll, waves, sb, groups, queue, prio, gq
+    ho = bool(candidate[7]) if len(candidate) > 7 else False
+    return unroll, waves, sb, groups, queue, prio, gq, ho
 
 
 def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
@@ -120,8 +131,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb, groups, queue, prio, gq = _knobs(cand)
-        opts = options(unroll, waves, base, sb, groups, queue, prio, gq)
+        unroll, waves, sb, groups, queue, prio, gq, ho = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -133,7 +144,7 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             times.append(ctx.timer_end())
         res = r.program.kernel_info("render_passes")
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                      "pixel_queue": queue, "wave_prio": prio, "group_queue": gq,
+                      "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho,
                       "ms": sorted(times)[len(times) // 2], "options": opts,
                       "vgprs": res["vgprs"], "scratch": res["scratch"]})
     best = pick(table, spill_tolerance)

@@ -916,6 +916,18 @@ bool g_count_culled = false;
 // also the rebuild of the ray's cull state for each (group, ray) pair.
 int g_count_group = 0;
 bool g_count_queue = false;
+// HT_HIT_ORDER (with the group queue): each passing (element, ray) pair's
+// crossing interval; exact tests only for the pairs the interval keeps.
+bool g_count_hit_order = false;
+
+// Crossing interval of a pair (scan.hh CullRayHy::interval): drop = no
+// crossing of the outer sphere ahead of the start; lb = e^D of the first
+// crossing ahead (1 if the start is inside the outer sphere), rounded down;
+// certain: the geodesic crosses the inner sphere ahead, no farther than ub.
+struct HitIv {
+    float lb = 1.0f, ub = 0.0f;
+    bool drop = false, certain = false;
+};
 
 struct CntD {
     double v;
@@ -984,13 +996,15 @@ template <class R> BoundF mo_bound(const C22<R> &m, int kind) {
 
 // Straight Euclidean ray p + t d, t >= 0, against a sphere.
 struct CullRayEu {
+    static constexpr bool HYPERBOLIC = false;
     CntD px, py, pz, dx, dy, dz;
+    CntD tp, ipz, iux, iuy;  // unused: straight rays have no crossing interval
     template <class R> static CullRayEu make(const V3<R> &p, const V3<R> &d) {
         return {CntD(tofloat(p.x)), CntD(tofloat(p.y)), CntD(tofloat(p.z)),
                 CntD(tofloat(d.x)), CntD(tofloat(d.y)), CntD(tofloat(d.z))};
     }
     // branch-free, as the kernel evaluates it: every term is computed
-    bool candidate(const BoundF &b) const {
+    bool candidate(const BoundF &b, bool = false) const {
         CntD sx = CntD(b.cx) - px, sy = CntD(b.cy) - py, sz = CntD(b.cz) - pz;
         CntD r = b.r;
         CntD t = sx * dx + sy * dy + sz * dz;
@@ -998,6 +1012,9 @@ struct CullRayEu {
         bool hit = !(t.v < -r.v) && (qx * qx + qy * qy + qz * qz).v <= (r * r).v;
         return b.r < 0.f || hit;
     }
+    // straight rays: no crossing interval
+    template <class R> void interval_consts(const V3<R> &, const V3<R> &) {}
+    HitIv interval(const BoundF &, bool) const { return HitIv(); }
 };
 
 // Hyperbolic geodesic from p with tangent d: the semicircle in the vertical
@@ -1008,17 +1025,18 @@ struct CullRayEu {
 // the behind-the-start case (the horosphere test accepts crossings behind
 // the start for near-vertical local rays, shape/hy/horosphere.cc:16-24).
 struct CullRayHy {
-    CntD px, py, pz, dx, dy, dz, dn2, ux, uy, gam, rg2, four_rg2;
+    static constexpr bool HYPERBOLIC = true;
+    CntD px, py, pz, dx, dy, dz, dn2, m, pz2, pz2n, four_pz2n, two_pzdz;
     bool valid = false;
-    // one square root and one division, as the kernel builds it
+    // no square root and no division, as the kernel builds it: the tests
+    // are multiplied through by m = |d.xy|^2 (see candidate)
     template <class R> static CullRayHy make(const V4<R> &p, const V4<R> &d) {
         CullRayHy c;
         CntD x = tofloat(d.x), y = tofloat(d.y), z = tofloat(d.z);
         CntD pz = tofloat(p.z);
-        CntD dxy2 = x * x + y * y;
-        CntD dn2 = dxy2 + z * z;
-        c.valid = dxy2.v > (CntD(1e-8) * dn2).v && pz.v > 0.0;
-        CntD inv = CntD(1.0) / sqrtd(c.valid ? dxy2 : CntD(1.0));
+        CntD m = x * x + y * y;
+        CntD dn2 = m + z * z;
+        c.valid = m.v > (CntD(1e-8) * dn2).v && pz.v > 0.0;
         c.px = tofloat(p.x);
         c.py = tofloat(p.y);
         c.pz = pz;
@@ -1026,12 +1044,80 @@ struct CullRayHy {
         c.dy = y;
         c.dz = z;
         c.dn2 = dn2;
-        c.ux = x * inv;
-        c.uy = y * inv;
-        c.gam = pz * z * inv;
-        c.rg2 = pz * pz * dn2 * (inv * inv);
-        c.four_rg2 = CntD(4.0) * c.rg2;
+        c.m = m;
+        c.pz2 = pz * pz;
+        c.pz2n = c.pz2 * dn2;
+        c.four_pz2n = CntD(4.0) * c.pz2n;
+        c.two_pzdz = CntD(2.0) * pz * z;
         return c;
+    }
+    // Crossing interval (scan.hh CullRayHy::interval, which derives it):
+    // with w = e^D along the ray, a sphere's crossings with the geodesic are
+    // the roots of gam w^2 - 2 bet w + alp = 0.
+    CntD tp, ipz, iux, iuy;  // the interval's per-ray constants (the kernel's hit_interval_consts)
+    template <class R> void interval_consts(const V4<R> &p, const V4<R> &d) {
+        CntD x = tofloat(d.x), y = tofloat(d.y), z = tofloat(d.z), pzz = tofloat(p.z);
+        CntD m = x * x + y * y, n = m + z * z;
+        bool ok = m.v > 1e-8 * n.v && pzz.v > 0.0;
+        CntD sm = sqrtd(ok ? m : CntD(1.0)), dn = sqrtd(ok ? n : CntD(1.0)), inv = CntD(1.0) / sm;
+        CntD t = z.v >= 0.0 ? (dn + z) * inv : sm / (dn - z);
+        tp = ok ? t : CntD(0.0);
+        ipz = ok ? CntD(1.0) / pzz : CntD(0.0);
+        iux = x * inv;
+        iuy = y * inv;
+    }
+    static float down(double x) {
+        float f = (float)x;
+        if ((double)f > x) f = next_down(f);
+        return f;
+    }
+    static float up(double x) {
+        float f = (float)x;
+        if ((double)f < x) f = next_up(f);
+        return f;
+    }
+    HitIv interval(const BoundF &b, bool exempt) const {
+        HitIv iv;
+        if (!(tp.v > 0.0) || exempt || b.r < 0.f) return iv;
+        CntD rx = CntD(b.cx) - px, ry = CntD(b.cy) - py;
+        CntD sig = rx * iux + ry * iuy, delta = CntD(0.0) - rx * iuy + ry * iux;
+        CntD d2p = delta * delta;
+        CntD S = sig * ipz, Z = CntD(b.cz) * ipz, ipz2 = ipz * ipz;
+        CntD po2 = (CntD(b.r) * CntD(b.r) - d2p) * ipz2;
+        CntD zm = CntD(1.0) - Z;
+        if ((S * S + zm * zm).v < po2.v) return iv;
+        if (!(po2.v > 0.0)) {
+            iv.drop = true;
+            return iv;
+        }
+        CntD tp2 = tp * tp;
+        CntD u = CntD(1.0) + S * tp, v = tp - S, zt = Z * tp;
+        CntD a0 = u * u + zt * zt, g0 = v * v + Z * Z, bet = (tp2 + CntD(1.0)) * Z;
+        CntD pi2 = (CntD(b.ri) * CntD(b.ri) - d2p) * ipz2;
+        if (pi2.v > 0.0) {
+            CntD ai = a0 - pi2 * tp2, gi = g0 - pi2;
+            CntD di = bet * bet - ai * gi;
+            if (di.v >= 0.0) {
+                CntD qi = bet + sqrtd(di);
+                if (qi.v > 0.0 && (gi.v <= 0.0 || qi.v >= gi.v * (1.0 + 1e-6))) {
+                    double ub = (ai / qi).v * (1.0 + 1e-9);
+                    if (ub >= 1.0 && ub < 1e30) {
+                        iv.certain = true;
+                        iv.ub = up(ub);
+                    }
+                }
+            }
+        }
+        CntD ao = a0 - po2 * tp2, go = g0 - po2;
+        CntD disc = bet * bet - ao * go;
+        CntD q = bet + sqrtd(disc.v > 0.0 ? disc : CntD(0.0));
+        if (!iv.certain && (disc.v < 0.0 || (go.v > 0.0 && q.v < go.v * (1.0 - 1e-6)))) {
+            iv.drop = true;
+            return iv;
+        }
+        double w = (ao / q).v;
+        iv.lb = w > 1.0 ? down(w * (1.0 - 1e-9)) : 1.0f;
+        return iv;
     }
     bool toward_tangent_point(double tx, double ty) const {
         CntD vx = CntD(tx) - px, vy = CntD(ty) - py;
@@ -1043,30 +1129,31 @@ struct CullRayHy {
     }
     // branch-free, as the kernel evaluates it: every term is computed; the
     // tangent-point test only for a horoball whose cut lies behind p
-    bool candidate(const BoundF &b) const {
+    // branch-free, as the kernel evaluates it: every term is computed; the
+    // tangent-point test only for a horoball whose cut lies behind p; a tree
+    // node (node = true) skips the inner-sphere test, which cannot cull it
+    bool candidate(const BoundF &b, bool node = false) const {
         CntD rx = CntD(b.cx) - px, ry = CntD(b.cy) - py;
-        CntD ro = b.r, ri = b.ri;
-        CntD delta = CntD(0.0) - rx * uy + ry * ux;
-        CntD d2p = delta * delta;
-        CntD rp2 = ro * ro - d2p;
-        CntD sig = rx * ux + ry * uy;
-        CntD ex = sig - gam;
-        CntD d2 = ex * ex + CntD(b.cz) * CntD(b.cz);
-        bool cull = rp2.v < 0.0;
-        bool behind = sig.v < 0.0 && (sig * sig).v > rp2.v;
+        CntD S = rx * dx + ry * dy;
+        CntD D = CntD(0.0) - rx * dy + ry * dx;
+        CntD d2 = D * D;
+        CntD rp = CntD(b.r) * CntD(b.r) * m - d2;
+        CntD s2 = S * S;
+        bool cull = rp.v < 0.0;
+        bool behind = S.v < 0.0 && s2.v > rp.v;
         if (behind && b.horo_node) behind = false;
         else if (behind && b.horo) behind = !toward_tangent_point(b.cx, b.cy);
         cull = cull || behind;
-        CntD o = d2 - rg2 - rp2;
-        CntD four_rr = four_rg2 * rp2;
-        cull = cull || (o.v > 0.0 && (o * o).v > four_rr.v);
-        CntD in1 = rg2 + rp2 - d2;
-        CntD in1sq = in1 * in1;
-        cull = cull || (rg2.v > rp2.v && in1.v > 0.0 && in1sq.v > four_rr.v);
-        CntD rpi2 = ri * ri - d2p;
-        CntD in2 = rg2 + rpi2 - d2;
-        CntD in2sq = in2 * in2, four_ri = four_rg2 * rpi2;
-        cull = cull || (rpi2.v > rg2.v && in2.v > 0.0 && in2sq.v > four_ri.v);
+        CntD o = s2 - two_pzdz * S + (CntD(b.cz) * CntD(b.cz) - pz2) * m - rp;
+        CntD o2 = o * o, f = four_pz2n * rp;
+        cull = cull || (o.v > 0.0 && o2.v > f.v);
+        cull = cull || (pz2n.v > rp.v && o.v < 0.0 && o2.v > f.v);
+        if (!node) {
+            CntD ri2 = CntD(b.ri) * CntD(b.ri);
+            CntD rpi = ri2 * m - d2;
+            CntD in2 = (ri2 - CntD(b.r) * CntD(b.r)) * m - o;
+            cull = cull || (rpi.v > pz2n.v && in2.v > 0.0 && (in2 * in2).v > (four_pz2n * rpi).v);
+        }
         return b.r < 0.f || !valid || !cull;
     }
 };
@@ -1212,8 +1299,11 @@ template <class R, class F, class G> struct VecObject : Object<R, F, G> {
     }
 
     // the candidate elements of a ray, in the order the scan finds them
-    template <class CR, class Ray_> void candidates(const CR &cr, const Ray_ &ray, std::vector<uint32_t> &out) const {
+    template <class CR, class Ray_> void candidates(const CR &cr, const Ray_ &ray, std::vector<uint32_t> &out,
+                                                    std::vector<HitIv> *ivs = nullptr, uint32_t orig = 0,
+                                                    uint32_t prev = 0) const {
         out.clear();
+        if (ivs) ivs->clear();
         if (group.empty()) {
             for (size_t i = 0; i < items.size(); ++i)
                 if (cr.candidate(bounds[i])) out.push_back((uint32_t)i);
@@ -1221,12 +1311,25 @@ template <class R, class F, class G> struct VecObject : Object<R, F, G> {
         }
         const size_t g = (size_t)g_count_group;
         for (size_t k = 0; k < group.size(); ++k) {
-            if (group_empty[k] || !cr.candidate(group[k])) continue;
+            if (group_empty[k] || !cr.candidate(group[k], true)) continue;
             CR c2 = cr;
             if (g_count_queue) c2 = cull_ray(ray);  // the pair's lane rebuilds the ray's cull state
+            if (ivs) c2.tp = cr.tp, c2.ipz = cr.ipz, c2.iux = cr.iux, c2.iuy = cr.iuy;  // read from the ray's slot
             for (size_t j = 0; j < g; ++j) {
                 uint32_t e = order[k * g + j];
-                if (e != 0xFFFFu && c2.candidate(bounds[e])) out.push_back(e);
+                if (e != 0xFFFFu && c2.candidate(bounds[e])) {
+                    if (ivs) {
+                        // the ray leaving this element: its test applies the repeat rule
+                        bool exempt = hash_step(orig, e) == prev;
+                        if constexpr (CR::HYPERBOLIC) {
+                            if (!exempt && bounds[e].horo) exempt = c2.toward_tangent_point(bounds[e].cx, bounds[e].cy);
+                        }
+                        HitIv iv = c2.interval(bounds[e], exempt);
+                        if (iv.drop) continue;
+                        ivs->push_back(iv);
+                    }
+                    out.push_back(e);
+                }
             }
         }
     }
@@ -1240,7 +1343,29 @@ template <class R, class F, class G> struct VecObject : Object<R, F, G> {
         R min_dist = R(-1);
         uint32_t min_index = 0;
         std::vector<uint32_t> cand;
-        candidates(cr, light.ray, cand);
+        if (g_count_hit_order && !group.empty()) {
+            // the per-ray interval constants, once per scan
+            cr.interval_consts(light.ray.start, light.ray.dir);
+            std::vector<HitIv> ivs;
+            candidates(cr, light.ray, cand, &ivs, orig, ctx.prev_hash);
+            // the ring's filter against the ray's nearest certain hit (the
+            // entry keeps the upper 16 bits of lb: rounded down)
+            float t = INFINITY;
+            for (const HitIv &iv : ivs)
+                if (iv.certain) t = std::min(t, iv.ub);
+            std::vector<uint32_t> kept;
+            for (size_t k = 0; k < cand.size(); ++k) {
+                uint32_t bits;
+                std::memcpy(&bits, &ivs[k].lb, 4);
+                bits &= 0xFFFF0000u;
+                float lbf;
+                std::memcpy(&lbf, &bits, 4);
+                if (!(lbf >= t * 1.0203f)) kept.push_back(cand[k]);
+            }
+            cand.swap(kept);
+        } else {
+            candidates(cr, light.ray, cand);
+        }
         std::sort(cand.begin(), cand.end());  // the reference loop's order decides ties
         for (uint32_t i : cand) {
             auto mo = static_cast<const MappedObject<R, F, G> *>(items[i].get());
@@ -1795,7 +1920,9 @@ int oracle_render(const char *spec, int precision, uint32_t w, uint32_t h, uint3
 // cull tests, exact tests of candidates, one map back per hit); culled = g
 // (2..16): the two-level scan over groups of g (group-sphere tests, then
 // the leaf tests of the passing groups); culled = 1000 + g: the same with
-// the group queue (the ray's cull state rebuilt per passing group).
+// the group queue (the ray's cull state rebuilt per passing group); culled
+// = 2000 + g: the group queue with the hit order (each passing pair's
+// crossing interval; exact tests only of the pairs its filter keeps).
 int oracle_count_flops(const char *spec, int culled, uint32_t w, uint32_t h, uint32_t passes, float *canvas,
                        uint32_t *seeds, uint32_t x0, uint32_t y0, uint32_t cw, uint32_t ch, int threads,
                        uint64_t *flops32, uint64_t *flops64) {
@@ -1803,12 +1930,14 @@ int oracle_count_flops(const char *spec, int culled, uint32_t w, uint32_t h, uin
         if (x0 + cw > w || y0 + ch > h) throw std::runtime_error("crop outside the image");
         g_count_culled = culled != 0;
         g_count_queue = culled >= 1000;
+        g_count_hit_order = culled >= 2000;
         g_count_group = culled >= 2 ? culled % 1000 : 0;
         uint64_t f64 = 0;
         uint64_t f32 = render_any<Cnt, Cnt>(spec, w, h, passes, canvas, seeds, x0, y0, cw, ch, threads, &f64);
         g_count_culled = false;
         g_count_group = 0;
         g_count_queue = false;
+        g_count_hit_order = false;
         if (flops32) *flops32 = f32;
         if (flops64) *flops64 = f64;
         return 0;
@@ -1816,6 +1945,7 @@ int oracle_count_flops(const char *spec, int culled, uint32_t w, uint32_t h, uin
         g_count_culled = false;
         g_count_group = 0;
         g_count_queue = false;
+        g_count_hit_order = false;
         g_err = e.what();
         return -1;
     }

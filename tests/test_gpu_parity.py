@@ -231,6 +231,14 @@ CULLED_VARIANTS = {
     "gq4_q_prio_w6": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_PIXEL_QUEUE=1"
                      " -DHT_WAVE_PRIO=1 -DHT_WAVES_PER_EU=6",
     "gq16_u2": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=16 -DHT_GROUP_QUEUE=1 -DHT_CULL_UNROLL=2",
+    "gq4_ho_prio": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_WAVE_PRIO=1"
+                   " -DHT_HIT_ORDER=1",
+    "gq4_ho_w3": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_WAVE_PRIO=1"
+                 " -DHT_HIT_ORDER=1 -DHT_WAVES_PER_EU=3",
+    "gq4_ho_q_prio": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_PIXEL_QUEUE=1"
+                     " -DHT_WAVE_PRIO=1 -DHT_HIT_ORDER=1",
+    "gq2_ho_u2": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=2 -DHT_GROUP_QUEUE=1 -DHT_CULL_UNROLL=2"
+                 " -DHT_HIT_ORDER=1",
 }
 
 
@@ -251,7 +259,10 @@ CULLED_VARIANTS = {
     ("hy_cells", True, "gq4_prio"), ("hy_cells", False, "gq8"), ("sdf_1024", False, "gq4_prio"),
     ("sdf_1024", False, "gq8"), ("eu_ball", False, "gq4_prio"), ("hy_example", False, "gq8"),
     ("eu_example", False, "gq4_prio"), ("hy_cells", True, "gq4_q_prio_w6"), ("sdf_1024", False, "gq4_q_prio_w6"),
-    ("hy_cells", False, "gq16_u2"), ("sdf", False, "gq8")])
+    ("hy_cells", False, "gq16_u2"), ("sdf", False, "gq8"),
+    ("hy_cells", True, "gq4_ho_prio"), ("hy_cells", False, "gq4_ho_w3"), ("hy_example", False, "gq4_ho_prio"),
+    ("eu_ball", False, "gq4_ho_prio"), ("sdf_1024", False, "gq4_ho_w3"), ("hy_cells", True, "gq4_ho_q_prio"),
+    ("eu_example", False, "gq4_ho_w3"), ("hy_cells", False, "gq2_ho_u2")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
@@ -271,7 +282,32 @@ def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6"])
+@pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3"])
+@pytest.mark.parametrize("frame", ["640x480", "1920x1080"])
+def test_hit_order_identical_on_full_frames(ctx, frame, variant):
+    """The hit-order scan (each passing pair's crossing interval: pairs that
+    never enter the element's sphere ahead of the start are dropped, pairs
+    beyond the ray's nearest certain hit skipped) on the benchmark frames:
+    the 192-primitive cell scene at 640x480 over 16 passes and the
+    256-primitive scene at 1920x1080 over 3 passes, canvas and seeds equal
+    to the element-by-element scan's."""
+    if frame == "640x480":
+        wl, passes = scenes.hyperbolic_cells(), 16
+    else:
+        wl, passes = scenes.hyperbolic_cells(1920, 1080, 1, primitives=256), 3
+    print(f"case hit order {variant} {frame} {passes} passes", flush=True)
+    buf = EntityBuffer(ctx, wl.scene)
+    out = []
+    for fp in (CULLED_VARIANTS[variant], NO_CULL):
+        r = Render(ctx, wl.scene, fp_options=fp)
+        c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
+        r.render_passes(buf, c, passes)
+        out.append((c.load_image(), c.load_seeds()))
+    np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio"])
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
 def test_pixel_queue_ragged_frames(ctx, size, variant):
     """The pixel-queue programs on frames smaller than a block or with a
@@ -333,7 +369,7 @@ def test_fp32_hit_window_option(ctx, ctx64):
 
 # A cell scene whose program has bounds capacity 64, and the variants run on it.
 CAPACITY_BASE = lambda: scenes.hyperbolic_cells(64, 48, primitives=48)  # noqa: E731
-CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue", "sb_g4_q_prio"]
+CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue", "sb_g4_q_prio", "gq4_ho_prio"]
 
 
 @pytest.mark.parametrize("variant", CAPACITY_VARIANTS)

@@ -180,3 +180,31 @@ def test_culled_flop_count_keeps_the_render():
                 assert 0 < g64 < c64, (wl.name, mode, g64, c64)
             else:
                 assert g64 == c64, (wl.name, mode)
+
+
+def test_hit_order_count_keeps_the_render():
+    """The hit-order scan's count (2000 + g: each passing pair's crossing
+    interval, exact tests only of the pairs its filter keeps) renders exactly
+    what the group-queue count and the plain scan render, on the cell scene
+    (192 primitives, the tree's groups), the 256-primitive cell scene and the
+    reference's hyperbolic example (fewer elements than the tree's threshold:
+    the flat scan, no interval), and it runs fewer exact tests (FP32 flops)
+    on the cell scenes."""
+    from hypertrace_amd import scenes
+
+    for wl, passes in ((scenes.hyperbolic_cells(160, 120), 3),
+                       (scenes.hyperbolic_cells(192, 108, 1, primitives=256), 2),
+                       (scenes.reference_hyperbolic_example(64, 48), 2),
+                       (scenes.euclidean_spheres(64, 48, 1), 2)):
+        w, h = wl.width, wl.height
+        spec = wl.scene.spec_text(X32_F32)
+        seeds = seed_fill(w * h, wl.seed_mode, wl.seed)
+        c0, s0, _, _ = oracle.count_flops(spec, w, h, passes, seeds, False)
+        cq, sq, q32, q64 = oracle.count_flops(spec, w, h, passes, seeds, 1004)
+        ch, sh, h32, h64 = oracle.count_flops(spec, w, h, passes, seeds, 2004)
+        assert np.array_equal(c0.view(np.uint32), ch.view(np.uint32)) and np.array_equal(s0, sh), wl.name
+        assert np.array_equal(cq.view(np.uint32), ch.view(np.uint32)), wl.name
+        if wl.scene.geo == "Hy" and len(wl.scene.object.items) >= 16:
+            assert h32 < 0.7 * q32 and h64 > q64, (wl.name, q32, h32, q64, h64)
+        else:
+            assert (h32, h64) == (q32, q64), wl.name

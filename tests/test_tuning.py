@@ -17,7 +17,7 @@ def test_candidates_distinct():
 
 
 def test_knob_defaults():
-    assert _knobs((4, 5)) == (4, 5, False, 0, False, False, False)
+    assert _knobs((4, 5)) == (4, 5, False, 0, False, False, False, False)
     assert candidate_options((4, 4, True, 4, False, True, True)).endswith(
         "-DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_WAVE_PRIO=1 -DHT_GROUP_QUEUE=1")
     assert options(4, 5) == candidate_options((4, 5, False, 0, False))
@@ -33,7 +33,7 @@ def test_knob_macros_are_read_by_the_device_sources():
     for c in CANDIDATES:
         names |= set(re.findall(r"-D(HT_[A-Z0-9_]+)=", candidate_options(c)))
     assert {"HT_CULL_UNROLL", "HT_WAVES_PER_EU", "HT_SCALAR_BOUNDS", "HT_CULL_GROUPS", "HT_PIXEL_QUEUE",
-            "HT_WAVE_PRIO", "HT_GROUP_QUEUE"} <= names
+            "HT_WAVE_PRIO", "HT_GROUP_QUEUE", "HT_HIT_ORDER"} <= names
     for name in names:
         assert re.search(rf"#\s*(ifdef|ifndef|if|elif)\b[^\n]*\b{name}\b", source), name
 
@@ -88,6 +88,10 @@ def test_bench_fixed_knobs_and_gpu_count_guard():
     assert bench.fixed_knobs("4,5,sb,p")["best"]["options"] == candidate_options((4, 5, True, 0, False, True))
     assert bench.fixed_knobs("2,5")["best"]["options"] == options(2, 5)
     assert bench.fixed_knobs("4,4,sb,g4,p,gq")["best"]["options"] == candidate_options((4, 4, True, 4, False, True, True))
+    assert bench.fixed_knobs("4,3,sb,g4,p,gq,ho")["best"]["options"] == candidate_options(
+        (4, 3, True, 4, False, True, True, True))
+    assert bench.count_mode(bench.fixed_knobs("4,3,sb,g4,p,gq,ho")["best"]) == 2004
+    assert bench.count_mode(bench.fixed_knobs("4,4,sb,g4,p,gq")["best"]) == 1004
     r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 2 and "one GPU only" in r.stderr
