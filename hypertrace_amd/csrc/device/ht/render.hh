@@ -140,6 +140,66 @@ HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ sce
         uint level = 3;
         __builtin_amdgcn_s_setprio(3);
 #endif
+#ifdef HT_HELPERS
+        // Program option HT_HELPERS: a lane whose pixel has all its samples
+        // stays in the loop and joins the scans of the wave's remaining rays
+        // without a ray of its own (TopScan::detect with has_ray = false),
+        // so their pair tests keep running on full lanes; the wave leaves
+        // the loop when no lane has samples left. The bounce is split
+        // around the scan so that every lane of the wave reaches it.
+        typedef typename S::Object T;
+        const u8 *obj = S::object(scene);
+        bool alive = true;
+        for (;;) {
+            if (alive && start) {
+                if (done == passes) {
+                    alive = false;
+                } else {
+                    start_path<S>(scene, &ctx, px, &light, &emission);
+                    hop = 0;
+                    start = false;
+                }
+            }
+            if (__ballot(alive) == 0) break;
+            bool end = false;
+#ifdef HT_RR_DEPTH
+            if (alive && hop >= HT_RR_DEPTH && russian_roulette(&ctx, &light.base)) end = true;
+#endif
+            bool ray = alive && !end;
+            if (ray) ctx.hasher = hash_init();
+            typename T::Cache cache;
+            real dist = TopScan<T>::detect(obj, &ctx, &cache, &light, ray);
+            if (ray) {
+                // the rest of bounce()
+                if (dist > (real)-0.5f) {
+                    if (!T::interact(obj, &ctx, &cache, &light, &emission)) end = true;
+                    else ctx.prev_hash = hash_finish(&ctx.hasher);
+                } else {
+                    S::bg(scene, &ctx, &light, &emission);
+                    end = true;
+                }
+            }
+            if (!alive) continue;
+            ++hop;
+            if (hop >= hops) end = true;
+            if (end) {
+                acc.x += emission.x;
+                acc.y += emission.y;
+                acc.z += emission.z;
+                acc.w += 1.0f;
+                ++done;
+                start = true;
+            }
+#ifdef HT_WAVE_PRIO
+            if (level > 0 && __ballot(done < passes * (4 - level) / 4) == 0) {
+                --level;
+                if (level == 2) __builtin_amdgcn_s_setprio(2);
+                else if (level == 1) __builtin_amdgcn_s_setprio(1);
+                else __builtin_amdgcn_s_setprio(0);
+            }
+#endif
+        }
+#else
         for (;;) {
             if (start) {
                 if (done == passes) break;
@@ -167,6 +227,7 @@ HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ sce
             }
 #endif
         }
+#endif
     }
     canvas[idx] = acc;
     seeds[idx] = ctx.rng.state;

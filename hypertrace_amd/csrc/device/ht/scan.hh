@@ -826,8 +826,11 @@ template <class T> struct TopScan {
     static HD void prepare(const u8 *) {}
     static HD void fill_scalar(const u8 *, BoundS *, uint, uint) {}
     static HD void fill_scalar_block(const u8 *, BoundS *) {}
+    // has_ray = false: a lane without a ray of its own (nothing to do here)
     template <class G>
-    static HD real detect(const u8 *self, Context *ctx, typename T::Cache *cache, Light<G> *light) {
+    static HD real detect(const u8 *self, Context *ctx, typename T::Cache *cache, Light<G> *light,
+                          bool has_ray = true) {
+        if (!has_ray) return (real)-1;
         return T::detect(self, ctx, cache, light);
     }
 };
@@ -1124,7 +1127,8 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     // push_owned(element, candidate, ray lane) appends to the exact tests'
     // ring, drain() runs its full batches.
     template <class CR, class PushO, class Drain>
-    static HD void walk_groups_queued(usize n, const CR &cr, const ConstBoundS *sb, WaveScratch<G> *ws, uint lane,
+    static HD void walk_groups_queued(usize n, bool has_ray, const CR &cr, const ConstBoundS *sb, WaveScratch<G> *ws,
+                                      uint lane,
                                       unsigned long long below, uint nact, uint arank, PushO &push_owned,
                                       Drain &drain) {
         constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP, FIRST = P / GS;
@@ -1175,7 +1179,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
 #pragma unroll
             for (int u = 0; u < CULL_UNROLL; ++u) {
                 BoundS q = load_bound(sb + P + k0 + u);
-                pass[u] = q.r != -2.0 && cr.template candidate<true>(q);  // -2: a node over padding only
+                pass[u] = has_ray && q.r != -2.0 && cr.template candidate<true>(q);  // -2: a node over padding only
             }
 #pragma unroll
             for (int u = 0; u < CULL_UNROLL; ++u) gpush(k0 + u, pass[u]);
@@ -1238,12 +1242,21 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     }
 #endif
 
-    static HD real detect(const u8 *self, Context *ctx, typename OV::Cache *cache, Light<G> *light) {
+    // has_ray = false (program option HT_HELPERS): a lane whose pixel is
+    // done joins the wave's scan without a ray of its own; it pushes no
+    // pairs but takes other rays' pairs in the batches, so the last rays of
+    // a wave get their pair tests on full lanes.
+    static HD real detect(const u8 *self, Context *ctx, typename OV::Cache *cache, Light<G> *light,
+                          bool has_ray = true) {
         if constexpr (!CULL) {
+            if (!has_ray) return -R1;
             return OV::detect(self, ctx, cache, light);
         } else {
             usize n = V::len(self);
-            if (n > (usize)BOUNDS_CAP) return OV::detect(self, ctx, cache, light);
+            if (n > (usize)BOUNDS_CAP) {
+                if (!has_ray) return -R1;
+                return OV::detect(self, ctx, cache, light);
+            }
 #ifdef HT_SCALAR_BOUNDS
             const ConstBoundS *sb = scalar_bounds_base();
             auto bound = [&](usize e) { return load_bound(sb + e); };
@@ -1285,7 +1298,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                         HIT_LB_FIELD(lb16) | ((uint)i << 6) | owner;
                 qcount += (uint)__popcll(m);
             };
-            auto push = [&](usize i, bool cand) { push_owned(i, cand, lane, HIT_LB_ONE); };
+            auto push = [&](usize i, bool cand) { push_owned(i, cand && has_ray, lane, HIT_LB_ONE); };
 #ifdef HT_HIT_ORDER
             // Drops the ring's pairs whose entry lies beyond their ray's
             // nearest certain hit by the margin (stable in-place compaction,
@@ -1362,7 +1375,8 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             };
             if (tree) {
 #if defined(HT_GROUP_QUEUE)
-                if constexpr (TREE_OK) walk_groups_queued(n, cr, sb, ws, lane, below, nact, arank, push_owned, drain);
+                if constexpr (TREE_OK)
+                    walk_groups_queued(n, has_ray, cr, sb, ws, lane, below, nact, arank, push_owned, drain);
 #elif defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS)
                 if constexpr (TREE_OK) walk_groups_scalar(n, cr, sb, push, drain);
 #elif defined(HT_CULL_GROUPS)
@@ -1400,6 +1414,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 process(self, ws, arank, qcount, qhead, tree);
             }
             wave_sync();
+            if (!has_ray) return -R1;
             unsigned long long best = ws->best[lane];
             if (best == ~0ull) {
                 ctx->hasher.state = orig;

@@ -28,7 +28,9 @@ the group queue) gives each passing pair the interval of hyperbolic
 distance at which its geodesic crosses the element's bound spheres, drops
 the pairs whose geodesic never enters the sphere ahead of the start, and
 skips those that lie beyond the ray's nearest certain hit, so most rays
-run one or two exact tests instead of five. Every
+run one or two exact tests instead of five; HT_HELPERS keeps the lanes
+whose pixels are done in the loop as helpers that take other rays' pair
+tests, so a wave's last rays are not tested on nearly empty lanes. Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -55,12 +57,14 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 4, True, True, True), (4, 6, True, 4, True, True, True),
     (4, 3, True, 4, False, True, True), (4, 3, True, 4, True, True, True),
     (4, 4, True, 4, False, True, True, True), (4, 3, True, 4, False, True, True, True),
-    (4, 4, True, 4, True, True, True, True), (4, 3, True, 4, True, True, True, True))
+    (4, 4, True, 4, True, True, True, True), (4, 3, True, 4, True, True, True, True),
+    (4, 4, True, 4, False, True, True, True, True), (4, 3, True, 4, False, True, True, True, True),
+    (4, 4, True, 4, False, True, True, False, True), (4, 4, True, 0, False, True, False, False, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
             pixel_queue: bool = False, wave_prio: bool = False, group_queue: bool = False,
-            hit_order: bool = False) -> str:
+            hit_order: bool = False, helpers: bool = False) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
@@ -74,10 +78,12 @@ def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool 
         opts += " -DHT_GROUP_QUEUE=1"
     if hit_order:
         opts += " -DHT_HIT_ORDER=1"
+    if helpers:
+        opts += " -DHT_HELPERS=1"
     return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool, bool]:
     unroll, waves = int(candidate[0]), int(candidate[1])
     sb = bool(candidate[2]) if len(candidate) > 2 else False
     groups = int(candidate[3]) if len(candidate) > 3 else 0
@@ -85,7 +91,8 @@ def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, 
     prio = bool(candidate[5]) if len(candidate) > 5 else False
     gq = bool(candidate[6]) if len(candidate) > 6 else False
     ho = bool(candidate[7]) if len(candidate) > 7 else False
-    return unroll, waves, sb, groups, queue, prio, gq, ho
+    hl = bool(candidate[8]) if len(candidate) > 8 else False
+    return unroll, waves, sb, groups, queue, prio, gq, ho, hl
 
 
 def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
@@ -131,8 +138,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb, groups, queue, prio, gq, ho = _knobs(cand)
-        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho)
+        unroll, waves, sb, groups, queue, prio, gq, ho, hl = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho, hl)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -144,7 +151,7 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
             times.append(ctx.timer_end())
         res = r.program.kernel_info("render_passes")
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
-                      "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho,
+                      "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho, "helpers": hl,
                       "ms": sorted(times)[len(times) // 2], "options": opts,
                       "vgprs": res["vgprs"], "scratch": res["scratch"]})
     best = pick(table, spill_tolerance)

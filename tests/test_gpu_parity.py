@@ -239,6 +239,12 @@ CULLED_VARIANTS = {
                      " -DHT_WAVE_PRIO=1 -DHT_HIT_ORDER=1",
     "gq2_ho_u2": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=2 -DHT_GROUP_QUEUE=1 -DHT_CULL_UNROLL=2"
                  " -DHT_HIT_ORDER=1",
+    "gq4_ho_hl": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_WAVE_PRIO=1"
+                 " -DHT_HIT_ORDER=1 -DHT_HELPERS=1",
+    "gq4_hl_w3": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_HELPERS=1"
+                 " -DHT_WAVES_PER_EU=3",
+    "sb_hl": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_HELPERS=1 -DHT_WAVE_PRIO=1",
+    "hl": STRICT_FP + " -DHT_HELPERS=1",
 }
 
 
@@ -262,7 +268,10 @@ CULLED_VARIANTS = {
     ("hy_cells", False, "gq16_u2"), ("sdf", False, "gq8"),
     ("hy_cells", True, "gq4_ho_prio"), ("hy_cells", False, "gq4_ho_w3"), ("hy_example", False, "gq4_ho_prio"),
     ("eu_ball", False, "gq4_ho_prio"), ("sdf_1024", False, "gq4_ho_w3"), ("hy_cells", True, "gq4_ho_q_prio"),
-    ("eu_example", False, "gq4_ho_w3"), ("hy_cells", False, "gq2_ho_u2")])
+    ("eu_example", False, "gq4_ho_w3"), ("hy_cells", False, "gq2_ho_u2"),
+    ("hy_cells", True, "gq4_ho_hl"), ("hy_cells", False, "gq4_hl_w3"), ("hy_example", False, "gq4_ho_hl"),
+    ("sdf_1024", False, "gq4_ho_hl"), ("eu_ball", False, "sb_hl"), ("eu_example", False, "hl"),
+    ("sdf", False, "hl"), ("eu_unit", False, "gq4_ho_hl")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
@@ -282,7 +291,25 @@ def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3"])
+@pytest.mark.parametrize("name", ["sdf_1024", "eu_ball"])
+def test_helpers_with_russian_roulette(ctx, name):
+    """Helper lanes (HT_HELPERS) with Russian roulette, where a path can end
+    before its scan: the same canvas and seeds as the same program without
+    helpers."""
+    wl = WORKLOADS[name]()
+    print(f"case helpers + roulette {name}", flush=True)
+    buf = EntityBuffer(ctx, wl.scene)
+    out = []
+    for fp in (program_options(rr_depth=3) + " -DHT_HELPERS=1", program_options(rr_depth=3)):
+        r = Render(ctx, wl.scene, fp_options=fp)
+        c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
+        r.render_passes(buf, c, 6)
+        out.append((c.load_image(), c.load_seeds()))
+    np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3", "gq4_ho_hl"])
 @pytest.mark.parametrize("frame", ["640x480", "1920x1080"])
 def test_hit_order_identical_on_full_frames(ctx, frame, variant):
     """The hit-order scan (each passing pair's crossing interval: pairs that
@@ -307,7 +334,7 @@ def test_hit_order_identical_on_full_frames(ctx, frame, variant):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio"])
+@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio", "gq4_ho_hl"])
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
 def test_pixel_queue_ragged_frames(ctx, size, variant):
     """The pixel-queue programs on frames smaller than a block or with a
@@ -369,7 +396,8 @@ def test_fp32_hit_window_option(ctx, ctx64):
 
 # A cell scene whose program has bounds capacity 64, and the variants run on it.
 CAPACITY_BASE = lambda: scenes.hyperbolic_cells(64, 48, primitives=48)  # noqa: E731
-CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue", "sb_g4_q_prio", "gq4_ho_prio"]
+CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue", "sb_g4_q_prio", "gq4_ho_prio",
+                     "gq4_ho_hl"]
 
 
 @pytest.mark.parametrize("variant", CAPACITY_VARIANTS)

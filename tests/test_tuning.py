@@ -17,7 +17,7 @@ def test_candidates_distinct():
 
 
 def test_knob_defaults():
-    assert _knobs((4, 5)) == (4, 5, False, 0, False, False, False, False)
+    assert _knobs((4, 5)) == (4, 5, False, 0, False, False, False, False, False)
     assert candidate_options((4, 4, True, 4, False, True, True)).endswith(
         "-DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_WAVE_PRIO=1 -DHT_GROUP_QUEUE=1")
     assert options(4, 5) == candidate_options((4, 5, False, 0, False))
