@@ -47,12 +47,13 @@ METRIC = "Mpaths/s + time-to-RMSE 0.01 vs FP64 oracle: chosen form vs plain kern
 FP32_VECTOR_PEAK = 157.3  # TFLOP/s, MI355X published FP32 vector (packed / dual-issue)
 FP32_NONPACKED_PEAK = 78.6  # TFLOP/s without packed FP32 (this project compiles packed FP32 off)
 # HBM bytes per render_passes launch on this workload with the tuner's pick
-# (scalar bounds, groups of 4 with the group queue, wave priority), from the
-# FETCH_SIZE / WRITE_SIZE counter runs scaled by the calibration copy
-# (FETCH x 2.00, WRITE x 1.00; profiles/render_passes_hyperbolic_cells_culled.md):
-# 10 852 KiB read + 10 096 KiB written, against 12.3 MB algorithmic; the
-# excess is the program's 20 B per lane of register spills.
-MEASURED_TRAFFIC_BYTES = (10852 + 10096) * 1024
+# (scalar bounds, groups of 4 with the group queue, wave priority, hit order,
+# helper lanes), from the FETCH_SIZE / WRITE_SIZE counter runs scaled by the
+# calibration copy (FETCH x 2.00, WRITE x 1.00;
+# profiles/render_passes_hyperbolic_cells_culled.md): 13 853 KiB read +
+# 14 287 KiB written, against 12.3 MB algorithmic; the excess is the
+# program's 36 B per lane of register spills.
+MEASURED_TRAFFIC_BYTES = (13853 + 14287) * 1024
 NO_CULL = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_NO_CULL=1"
 ORACLE_SEED_XOR = 0x0BAD5EED
 
