@@ -683,7 +683,7 @@ def batch_leg(args):
     specs = scenes.batch_jobs()
     paths = sum(s.width * s.height * s.spp for s in specs)
     with Context(0, X32_F32) as ctx:
-        tune = autotune(ctx, specs, reps=1)
+        tune = autotune(ctx, specs, reps=1, progress=lambda m: print(m, file=sys.stderr, flush=True))
         best = tune["best"]
         bprog = BatchProgram(ctx, best["block"], fp_options=best.get("options"))
         batch = Batch(ctx, bprog, specs, best["tile"], with_rgba8=True)

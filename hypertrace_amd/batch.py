@@ -160,11 +160,12 @@ def tile_unit(opts: Optional[str], block: int) -> int:
 
 def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mults=TILE_MULTS,
              grid_waves_per_cu=(8, 16), reps: int = 2, cache_path: Optional[str] = None,
-             options=OPTIONS) -> Dict:
+             options=OPTIONS, progress=None) -> Dict:
     """Times one frame of `specs` for every (program options, block, tile,
     grid) candidate and returns the fastest with the whole table. grid =
     blocks such that each CU gets the given number of waves; `options` pairs
-    program options with the block sizes they are tried at."""
+    program options with the block sizes they are tried at; `progress`, if
+    given, is called with a line of text after each program's candidates."""
     key = f"gfx950:{_signature(specs)}"
     if cache_path and os.path.exists(cache_path):
         with open(cache_path) as f:
@@ -190,6 +191,8 @@ def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mu
                     times.append(time.perf_counter() - t0)
                 table.append({"block": block, "tile": tile, "grid": grid, "options": opts, "ms": 1e3 * min(times)})
             batch.close()
+        if progress is not None:
+            progress(f"batch autotune: block {block} [{opts}]: best {min(r['ms'] for r in table[-len(tile_mults) * len(grid_waves_per_cu):]):.1f} ms")
     best = min(table, key=lambda r: r["ms"])
     result = {"best": best, "table": table, "paths": sum(s.width * s.height * s.spp for s in specs)}
     if cache_path:

@@ -1127,8 +1127,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     // push_owned(element, candidate, ray lane) appends to the exact tests'
     // ring, drain() runs its full batches.
     template <class CR, class PushO, class Drain>
-    static HD void walk_groups_queued(usize n, bool has_ray, const CR &cr, const ConstBoundS *sb, WaveScratch<G> *ws,
-                                      uint lane,
+    static HD void walk_groups_queued(usize n, bool has_ray, const ConstBoundS *sb, WaveScratch<G> *ws, uint lane,
                                       unsigned long long below, uint nact, uint arank, PushO &push_owned,
                                       Drain &drain) {
         constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP, FIRST = P / GS;
@@ -1153,8 +1152,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             }
             ghead = gring_wrap(ghead + k);
             gcount -= k;
-            CR c2 = cr;
-            if (have) c2 = CR::make(ws->ray[L]);
+            const CR c2 = CR::make(ws->ray[have ? L : lane]);
 #pragma unroll
             for (uint j = 0; j < GS; ++j) {
                 bool cand = false;
@@ -1175,11 +1173,15 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         // element are padding only
         const uint groups = ((uint)n + GS - 1) / GS;
         for (uint k0 = 0; k0 < groups; k0 += CULL_UNROLL) {
+            // the lane's own cull state, rebuilt from its ray in LDS at each
+            // step (a few multiplications): it is then not live in registers
+            // across the level-2 batches and the exact tests they drain
+            const CR c1 = CR::make(ws->ray[lane]);
             bool pass[CULL_UNROLL];
 #pragma unroll
             for (int u = 0; u < CULL_UNROLL; ++u) {
                 BoundS q = load_bound(sb + P + k0 + u);
-                pass[u] = has_ray && q.r != -2.0 && cr.template candidate<true>(q);  // -2: a node over padding only
+                pass[u] = has_ray && q.r != -2.0 && c1.template candidate<true>(q);  // -2: a node over padding only
             }
 #pragma unroll
             for (int u = 0; u < CULL_UNROLL; ++u) gpush(k0 + u, pass[u]);
@@ -1376,7 +1378,8 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             if (tree) {
 #if defined(HT_GROUP_QUEUE)
                 if constexpr (TREE_OK)
-                    walk_groups_queued(n, has_ray, cr, sb, ws, lane, below, nact, arank, push_owned, drain);
+                    walk_groups_queued<typename CullRayOf<G>::type>(n, has_ray, sb, ws, lane, below, nact, arank,
+                                                                   push_owned, drain);
 #elif defined(HT_CULL_GROUPS) && defined(HT_SCALAR_BOUNDS)
                 if constexpr (TREE_OK) walk_groups_scalar(n, cr, sb, push, drain);
 #elif defined(HT_CULL_GROUPS)
