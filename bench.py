@@ -25,10 +25,10 @@ region. Beside it:
   rows of the same frame, on the host's cores.
 
 The whole result (tuner tables, RMSE curves, every leg) goes to a JSON
-file (--detail, default gpurun_out/bench_detail.json); the last line of
-stdout is a summary of at most 2 KB: the headline, its roofline, the CPU
-baseline, the time-to-RMSE figures and one number per leg. Progress lines go
-to stderr.
+file (--detail; by default bench_detail.json in the run's output folder);
+the last line of stdout is a summary of at most 2 KB: the headline, its
+roofline, the CPU baseline, the time-to-RMSE figures and one number per
+leg. Progress lines go to stderr.
 """
 from __future__ import annotations
 
