@@ -540,7 +540,7 @@ def spp_for(fit, target):
 
 
 def fixed_knobs(text, base=None):
-    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl]" (no timing)."""
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN]" (no timing)."""
     from hypertrace_amd.processing import STRICT_FP
     from hypertrace_amd.tuning import options
     knobs = text.split(",")
@@ -553,11 +553,13 @@ def fixed_knobs(text, base=None):
     gq = "gq" in flags
     ho = "ho" in flags
     hl = "hl" in flags
+    sparse = next((int(k[1:]) for k in flags if k.startswith("s") and k[1:].isdigit()), 0)
     return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
                      "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho, "helpers": hl,
+                     "sparse": sparse,
                      "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups,
                                         pixel_queue=queue, wave_prio=prio, group_queue=gq, hit_order=ho,
-                                        helpers=hl)},
+                                        helpers=hl, sparse=sparse)},
             "table": [], "fixed": True}
 
 

@@ -152,7 +152,7 @@ def _suite_jobs():
         eu = scenes.euclidean_spheres(96, 72).scene
         jobs += [(scenes.euclidean_spheres(w, h).scene, X32_F32, par.CULLED_VARIANTS[v], None)
                  for (w, h) in ((1, 1), (64, 48)) for v in ("pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio",
-                                                           "gq4_ho_hl")]
+                                                           "gq4_ho_hl", "gq4_ho_q_s16")]
         jobs += [(eu, X32_F32, program_options(eps32=1e-5), None),
                  (eu, X32_F32, program_options(eps32=1e-5, no_cull=True), None)]
     bt = load("test_gpu_batch")

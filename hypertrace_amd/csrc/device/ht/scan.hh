@@ -89,6 +89,9 @@ HD uint gring_wrap(uint i) { return i >= GROUP_RING ? i - GROUP_RING : i; }
 #if defined(HT_GROUP_QUEUE) && defined(HT_BATCH_BLOCK)
 #error "HT_GROUP_QUEUE is an option of scene programs"
 #endif
+#if defined(HT_SPARSE_L1) && !defined(HT_GROUP_QUEUE)
+#error "HT_SPARSE_L1 is an option of the group queue (HT_GROUP_QUEUE)"
+#endif
 // Program option HT_HIT_ORDER (with HT_GROUP_QUEUE): before the exact tests,
 // each passing (element, ray) pair gets the interval of the ray's
 // hyperbolic distance at which its geodesic crosses the element's outer
@@ -680,6 +683,9 @@ template <class G> struct WaveScratch {
     uint tw[WAVE];       // float bits of the ray's nearest certain hit e^D (+inf: none yet)
     float4 ivc[WAVE];     // the ray's interval constants (hit_interval_consts)
 #endif
+#ifdef HT_SPARSE_L1
+    unsigned char rlane[WAVE];  // lane of the wave's j-th ray (sparse level 1)
+#endif
 };
 
 template <class Tag> HD Bound *bounds_lds() {
@@ -1134,10 +1140,10 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         static_assert(FIRST % CULL_UNROLL == 0, "level-1 steps cover the group nodes");
         const Bound *leaves = bounds_lds<OV>();
         uint gcount = 0, ghead = 0;
-        auto gpush = [&](uint k, bool pass) {
+        auto gpush = [&](uint k, bool pass, uint owner) {
             unsigned long long m = __ballot(pass);
             if (m == 0) return;
-            if (pass) ws->gqueue[gring_wrap(ghead + gcount + (uint)__popcll(m & below))] = (k << 6) | lane;
+            if (pass) ws->gqueue[gring_wrap(ghead + gcount + (uint)__popcll(m & below))] = (k << 6) | owner;
             gcount += (uint)__popcll(m);
         };
         // one batch of k <= nact (group, ray) pairs from the head of the ring
@@ -1172,6 +1178,45 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
         // then the unbounded): the groups after the last one holding an
         // element are padding only
         const uint groups = ((uint)n + GS - 1) / GS;
+#ifdef HT_SPARSE_L1
+        // A wave with few rays (program option HT_SPARSE_L1 = the most rays
+        // this applies to) runs level 1 over its (group, ray) pairs, one
+        // pair per active lane: groups * R pairs in ceil(groups * R / nact)
+        // steps instead of every group in every lane. Ray-minor order: the
+        // lanes of a step mostly read the same group's bound (vector loads
+        // of the array the other form reads through the scalar cache).
+        {
+            const unsigned long long live = __ballot(has_ray);
+            const uint R = (uint)__popcll(live);
+            if (R <= (uint)HT_SPARSE_L1) {
+                if (R == 0) return;
+                if (has_ray) ws->rlane[__popcll(live & below)] = (unsigned char)lane;  // the wave's j-th ray
+                wave_sync();
+                const uint qs = nact / R, rs = nact % R;
+                uint r = arank % R, g = arank / R;  // pair base + arank = g * R + r
+                const BoundS *nodes = ht_scalar_bounds + P;
+                for (uint base = 0; base < groups * R; base += nact) {
+                    const uint L = ws->rlane[r];
+                    bool pass = false;
+                    if (g < groups) {
+                        const CR c1 = CR::make(ws->ray[L]);
+                        const BoundS q = nodes[g];
+                        pass = q.r != -2.0 && c1.template candidate<true>(q);  // -2: a node over padding only
+                    }
+                    gpush(g, pass, L);
+                    while (gcount >= nact) gbatch(nact);
+                    r += rs;
+                    g += qs;
+                    if (r >= R) {
+                        r -= R;
+                        g += 1;
+                    }
+                }
+                if (gcount > 0) gbatch(gcount);
+                return;
+            }
+        }
+#endif
         for (uint k0 = 0; k0 < groups; k0 += CULL_UNROLL) {
             // the lane's own cull state, rebuilt from its ray in LDS at each
             // step (a few multiplications): it is then not live in registers
@@ -1184,7 +1229,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 pass[u] = has_ray && q.r != -2.0 && c1.template candidate<true>(q);  // -2: a node over padding only
             }
 #pragma unroll
-            for (int u = 0; u < CULL_UNROLL; ++u) gpush(k0 + u, pass[u]);
+            for (int u = 0; u < CULL_UNROLL; ++u) gpush(k0 + u, pass[u], lane);
             while (gcount >= nact) gbatch(nact);
         }
         if (gcount > 0) gbatch(gcount);

@@ -30,7 +30,10 @@ the pairs whose geodesic never enters the sphere ahead of the start, and
 skips those that lie beyond the ray's nearest certain hit, so most rays
 run one or two exact tests instead of five; HT_HELPERS keeps the lanes
 whose pixels are done in the loop as helpers that take other rays' pair
-tests, so a wave's last rays are not tested on nearly empty lanes. Every
+tests, so a wave's last rays are not tested on nearly empty lanes;
+HT_SPARSE_L1 = R (with the group queue) runs the group tests of a scan
+with at most R rays over its (group, ray) pairs, one pair per lane, instead
+of every group in every lane. Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -59,12 +62,14 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 4, False, True, True, True), (4, 3, True, 4, False, True, True, True),
     (4, 4, True, 4, True, True, True, True), (4, 3, True, 4, True, True, True, True),
     (4, 4, True, 4, False, True, True, True, True), (4, 3, True, 4, False, True, True, True, True),
-    (4, 4, True, 4, False, True, True, False, True), (4, 4, True, 0, False, True, False, False, True))
+    (4, 4, True, 4, False, True, True, False, True), (4, 4, True, 0, False, True, False, False, True),
+    (4, 4, True, 4, False, True, True, True, True, 32), (4, 4, True, 4, False, True, True, True, True, 16),
+    (4, 3, True, 4, False, True, True, True, True, 32))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
             pixel_queue: bool = False, wave_prio: bool = False, group_queue: bool = False,
-            hit_order: bool = False, helpers: bool = False) -> str:
+            hit_order: bool = False, helpers: bool = False, sparse: int = 0) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
@@ -80,10 +85,12 @@ def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool 
         opts += " -DHT_HIT_ORDER=1"
     if helpers:
         opts += " -DHT_HELPERS=1"
+    if sparse:
+        opts += f" -DHT_SPARSE_L1={int(sparse)}"
     return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool, bool]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool, bool, int]:
     unroll, waves = int(candidate[0]), int(candidate[1])
     sb = bool(candidate[2]) if len(candidate) > 2 else False
     groups = int(candidate[3]) if len(candidate) > 3 else 0
@@ -92,7 +99,8 @@ def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, 
     gq = bool(candidate[6]) if len(candidate) > 6 else False
     ho = bool(candidate[7]) if len(candidate) > 7 else False
     hl = bool(candidate[8]) if len(candidate) > 8 else False
-    return unroll, waves, sb, groups, queue, prio, gq, ho, hl
+    sparse = int(candidate[9]) if len(candidate) > 9 else 0
+    return unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse
 
 
 def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
@@ -138,8 +146,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb, groups, queue, prio, gq, ho, hl = _knobs(cand)
-        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho, hl)
+        unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho, hl, sparse)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -152,7 +160,7 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
         res = r.program.kernel_info("render_passes")
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
                       "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho, "helpers": hl,
-                      "ms": sorted(times)[len(times) // 2], "options": opts,
+                      "sparse": sparse, "ms": sorted(times)[len(times) // 2], "options": opts,
                       "vgprs": res["vgprs"], "scratch": res["scratch"]})
     best = pick(table, spill_tolerance)
     result = {"best": best, "table": table, "fastest_any": min(table, key=lambda t: t["ms"]),

@@ -245,6 +245,16 @@ CULLED_VARIANTS = {
                  " -DHT_WAVES_PER_EU=3",
     "sb_hl": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_HELPERS=1 -DHT_WAVE_PRIO=1",
     "hl": STRICT_FP + " -DHT_HELPERS=1",
+    # level 1 over (group, ray) pairs in waves with few rays (HT_SPARSE_L1):
+    # with helper lanes (all 64 lanes in the scan), with the pixel queue
+    # (only the lanes holding a pixel), and always on (64)
+    "gq4_ho_hl_s32": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_WAVE_PRIO=1"
+                     " -DHT_HIT_ORDER=1 -DHT_HELPERS=1 -DHT_SPARSE_L1=32",
+    "gq4_ho_q_s16": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_PIXEL_QUEUE=1"
+                    " -DHT_WAVE_PRIO=1 -DHT_HIT_ORDER=1 -DHT_SPARSE_L1=16",
+    "gq4_s64": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_SPARSE_L1=64",
+    "gq8_s8_u2": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=8 -DHT_GROUP_QUEUE=1 -DHT_CULL_UNROLL=2"
+                 " -DHT_SPARSE_L1=8",
 }
 
 
@@ -271,7 +281,10 @@ CULLED_VARIANTS = {
     ("eu_example", False, "gq4_ho_w3"), ("hy_cells", False, "gq2_ho_u2"),
     ("hy_cells", True, "gq4_ho_hl"), ("hy_cells", False, "gq4_hl_w3"), ("hy_example", False, "gq4_ho_hl"),
     ("sdf_1024", False, "gq4_ho_hl"), ("eu_ball", False, "sb_hl"), ("eu_example", False, "hl"),
-    ("sdf", False, "hl"), ("eu_unit", False, "gq4_ho_hl")])
+    ("sdf", False, "hl"), ("eu_unit", False, "gq4_ho_hl"),
+    ("hy_cells", True, "gq4_ho_hl_s32"), ("hy_cells", True, "gq4_ho_q_s16"), ("hy_cells", False, "gq4_s64"),
+    ("sdf_1024", False, "gq4_s64"), ("sdf_1024", False, "gq4_ho_hl_s32"), ("eu_ball", False, "gq4_ho_q_s16"),
+    ("hy_example", False, "gq4_ho_q_s16"), ("eu_example", False, "gq8_s8_u2"), ("hy_cells", False, "gq8_s8_u2")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
@@ -309,7 +322,7 @@ def test_helpers_with_russian_roulette(ctx, name):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3", "gq4_ho_hl"])
+@pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3", "gq4_ho_hl", "gq4_ho_hl_s32", "gq4_ho_q_s16"])
 @pytest.mark.parametrize("frame", ["640x480", "1920x1080"])
 def test_hit_order_identical_on_full_frames(ctx, frame, variant):
     """The hit-order scan (each passing pair's crossing interval: pairs that
@@ -334,7 +347,8 @@ def test_hit_order_identical_on_full_frames(ctx, frame, variant):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio", "gq4_ho_hl"])
+@pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio", "gq4_ho_hl",
+                                     "gq4_ho_q_s16"])
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
 def test_pixel_queue_ragged_frames(ctx, size, variant):
     """The pixel-queue programs on frames smaller than a block or with a
@@ -397,7 +411,7 @@ def test_fp32_hit_window_option(ctx, ctx64):
 # A cell scene whose program has bounds capacity 64, and the variants run on it.
 CAPACITY_BASE = lambda: scenes.hyperbolic_cells(64, 48, primitives=48)  # noqa: E731
 CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue", "sb_g4_q_prio", "gq4_ho_prio",
-                     "gq4_ho_hl"]
+                     "gq4_ho_hl", "gq4_ho_hl_s32", "gq4_s64"]
 
 
 @pytest.mark.parametrize("variant", CAPACITY_VARIANTS)
