@@ -145,14 +145,15 @@ def _suite_jobs():
                  for name in par.HY_DIST_FP64_CASES]
         rr = program_options(rr_depth=3)
         jobs += [(par.WORKLOADS[name]().scene, X32_F32, fp, None)
-                 for name in ("sdf_1024", "eu_ball") for fp in (rr, rr + " -DHT_HELPERS=1")]
+                 for name in ("sdf_1024", "eu_ball") for fp in [rr] + [rr + o for o in par.HELPER_RR_OPTIONS.values()]]
         cap = par.CAPACITY_BASE().scene
         jobs += [(cap, X32_F32, par.CULLED_VARIANTS[v], None) for v in par.CAPACITY_VARIANTS]
         jobs.append((cap, X32_F32, NO_CULL_FP, None))
         eu = scenes.euclidean_spheres(96, 72).scene
         jobs += [(scenes.euclidean_spheres(w, h).scene, X32_F32, par.CULLED_VARIANTS[v], None)
                  for (w, h) in ((1, 1), (64, 48)) for v in ("pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio",
-                                                           "gq4_ho_hl", "gq4_ho_q_s16")]
+                                                           "gq4_ho_hl", "gq4_ho_q_s16", "gq4_ho_q_hl_s16",
+                                                           "q_hl")]
         jobs += [(eu, X32_F32, program_options(eps32=1e-5), None),
                  (eu, X32_F32, program_options(eps32=1e-5, no_cull=True), None)]
     bt = load("test_gpu_batch")

@@ -286,20 +286,62 @@ template <class S> struct QueuedPixel {
         bool end = bounce<S>(scene, &ctx, &light, &emission, hop);
         ++hop;
         if (hop >= hops) end = true;
-        if (end) {
-            acc.x += emission.x;
-            acc.y += emission.y;
-            acc.z += emission.z;
-            acc.w += 1.0f;
-            ++done;
-            start = true;
-            if (done == passes) {
-                canvas[idx] = acc;
-                seeds[idx] = ctx.rng.state;
-                have = false;
-            }
+        if (end) finish_sample(canvas, seeds, passes);
+    }
+
+    HD void finish_sample(float4 *__restrict__ canvas, uint *__restrict__ seeds, uint passes) {
+        acc.x += emission.x;
+        acc.y += emission.y;
+        acc.z += emission.z;
+        acc.w += 1.0f;
+        ++done;
+        start = true;
+        if (done == passes) {
+            canvas[idx] = acc;
+            seeds[idx] = ctx.rng.state;
+            have = false;
         }
     }
+
+#ifdef HT_HELPERS
+    // step() for every lane of the wave (program option HT_HELPERS): a lane
+    // without a pixel joins the scan without a ray of its own and takes
+    // other rays' pair tests. The bounce is split around the scan so that
+    // every lane reaches it; a busy lane computes what step() computes.
+    HD void step_all(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
+                     uint *__restrict__ seeds, uint passes, int hops) {
+        typedef typename S::Object T;
+        const bool busy = have;
+        if (busy && start) {
+            start_path<S>(scene, &ctx, pixel_setup(idx % w, idx / w, w, h), &light, &emission);
+            hop = 0;
+            start = false;
+        }
+        bool end = false;
+#ifdef HT_RR_DEPTH
+        if (busy && hop >= HT_RR_DEPTH && russian_roulette(&ctx, &light.base)) end = true;
+#endif
+        const bool ray = busy && !end;
+        if (ray) ctx.hasher = hash_init();
+        const u8 *obj = S::object(scene);
+        typename T::Cache cache;
+        real dist = TopScan<T>::detect(obj, &ctx, &cache, &light, ray);
+        if (ray) {
+            // the rest of bounce()
+            if (dist > (real)-0.5f) {
+                if (!T::interact(obj, &ctx, &cache, &light, &emission)) end = true;
+                else ctx.prev_hash = hash_finish(&ctx.hasher);
+            } else {
+                S::bg(scene, &ctx, &light, &emission);
+                end = true;
+            }
+        }
+        if (!busy) return;
+        ++hop;
+        if (hop >= hops) end = true;
+        if (end) finish_sample(canvas, seeds, passes);
+    }
+#endif
 };
 
 #ifdef HT_PIXEL_QUEUE
@@ -361,7 +403,11 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
             if (drained) break;
             continue;
         }
+#ifdef HT_HELPERS
+        p.step_all(w, h, scene, canvas, seeds, passes, S::HOPS);  // lanes without a pixel help the scan
+#else
         if (p.have) p.step(w, h, scene, canvas, seeds, passes, S::HOPS);
+#endif
     }
 }
 #endif

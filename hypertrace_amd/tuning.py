@@ -29,7 +29,8 @@ distance at which its geodesic crosses the element's bound spheres, drops
 the pairs whose geodesic never enters the sphere ahead of the start, and
 skips those that lie beyond the ray's nearest certain hit, so most rays
 run one or two exact tests instead of five; HT_HELPERS keeps the lanes
-whose pixels are done in the loop as helpers that take other rays' pair
+whose pixels are done (with the pixel queue: the lanes without a pixel
+once the queue is empty) in the loop as helpers that take other rays' pair
 tests, so a wave's last rays are not tested on nearly empty lanes;
 HT_SPARSE_L1 = R (with the group queue) runs the group tests of a scan
 with at most R rays over its (group, ray) pairs, one pair per lane, instead
@@ -64,7 +65,9 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 4, False, True, True, True, True), (4, 3, True, 4, False, True, True, True, True),
     (4, 4, True, 4, False, True, True, False, True), (4, 4, True, 0, False, True, False, False, True),
     (4, 4, True, 4, False, True, True, True, True, 32), (4, 4, True, 4, False, True, True, True, True, 16),
-    (4, 3, True, 4, False, True, True, True, True, 32))
+    (4, 3, True, 4, False, True, True, True, True, 32),
+    (4, 4, True, 4, True, True, True, True, True), (4, 4, True, 4, True, True, True, True, True, 16),
+    (4, 4, True, 4, True, True, True, False, True), (4, 4, True, 4, True, True, True, False, True, 16))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,

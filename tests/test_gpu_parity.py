@@ -255,6 +255,13 @@ CULLED_VARIANTS = {
     "gq4_s64": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_SPARSE_L1=64",
     "gq8_s8_u2": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=8 -DHT_GROUP_QUEUE=1 -DHT_CULL_UNROLL=2"
                  " -DHT_SPARSE_L1=8",
+    # the pixel queue with helper lanes (the lanes without a pixel once the
+    # queue is empty join the scans)
+    "gq4_ho_q_hl_s16": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_PIXEL_QUEUE=1"
+                       " -DHT_WAVE_PRIO=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1 -DHT_SPARSE_L1=16",
+    "gq4_q_hl": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_PIXEL_QUEUE=1"
+                " -DHT_HELPERS=1",
+    "q_hl": STRICT_FP + " -DHT_PIXEL_QUEUE=1 -DHT_HELPERS=1",
 }
 
 
@@ -284,7 +291,9 @@ CULLED_VARIANTS = {
     ("sdf", False, "hl"), ("eu_unit", False, "gq4_ho_hl"),
     ("hy_cells", True, "gq4_ho_hl_s32"), ("hy_cells", True, "gq4_ho_q_s16"), ("hy_cells", False, "gq4_s64"),
     ("sdf_1024", False, "gq4_s64"), ("sdf_1024", False, "gq4_ho_hl_s32"), ("eu_ball", False, "gq4_ho_q_s16"),
-    ("hy_example", False, "gq4_ho_q_s16"), ("eu_example", False, "gq8_s8_u2"), ("hy_cells", False, "gq8_s8_u2")])
+    ("hy_example", False, "gq4_ho_q_s16"), ("eu_example", False, "gq8_s8_u2"), ("hy_cells", False, "gq8_s8_u2"),
+    ("hy_cells", True, "gq4_ho_q_hl_s16"), ("sdf_1024", False, "gq4_q_hl"), ("eu_ball", False, "gq4_ho_q_hl_s16"),
+    ("hy_example", False, "gq4_q_hl"), ("sdf", False, "q_hl"), ("eu_example", False, "q_hl")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
@@ -304,16 +313,22 @@ def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+HELPER_RR_OPTIONS = {"hl": " -DHT_HELPERS=1",
+                     "q_gq_hl_s16": " -DHT_PIXEL_QUEUE=1 -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1"
+                                    " -DHT_HELPERS=1 -DHT_SPARSE_L1=16"}
+
+
+@pytest.mark.parametrize("helpers", list(HELPER_RR_OPTIONS))
 @pytest.mark.parametrize("name", ["sdf_1024", "eu_ball"])
-def test_helpers_with_russian_roulette(ctx, name):
-    """Helper lanes (HT_HELPERS) with Russian roulette, where a path can end
-    before its scan: the same canvas and seeds as the same program without
-    helpers."""
+def test_helpers_with_russian_roulette(ctx, name, helpers):
+    """Helper lanes (HT_HELPERS; also with the pixel queue, whose lanes
+    without a pixel help) with Russian roulette, where a path can end before
+    its scan: the same canvas and seeds as the program without helpers."""
     wl = WORKLOADS[name]()
-    print(f"case helpers + roulette {name}", flush=True)
+    print(f"case helpers + roulette {name} {helpers}", flush=True)
     buf = EntityBuffer(ctx, wl.scene)
     out = []
-    for fp in (program_options(rr_depth=3) + " -DHT_HELPERS=1", program_options(rr_depth=3)):
+    for fp in (program_options(rr_depth=3) + HELPER_RR_OPTIONS[helpers], program_options(rr_depth=3)):
         r = Render(ctx, wl.scene, fp_options=fp)
         c = Canvas(ctx, (wl.width, wl.height), wl.seed_mode, wl.seed)
         r.render_passes(buf, c, 6)
@@ -322,7 +337,8 @@ def test_helpers_with_russian_roulette(ctx, name):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3", "gq4_ho_hl", "gq4_ho_hl_s32", "gq4_ho_q_s16"])
+@pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3", "gq4_ho_hl", "gq4_ho_hl_s32", "gq4_ho_q_s16",
+                                     "gq4_ho_q_hl_s16"])
 @pytest.mark.parametrize("frame", ["640x480", "1920x1080"])
 def test_hit_order_identical_on_full_frames(ctx, frame, variant):
     """The hit-order scan (each passing pair's crossing interval: pairs that
@@ -348,7 +364,7 @@ def test_hit_order_identical_on_full_frames(ctx, frame, variant):
 
 
 @pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio", "gq4_ho_hl",
-                                     "gq4_ho_q_s16"])
+                                     "gq4_ho_q_s16", "gq4_ho_q_hl_s16", "q_hl"])
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
 def test_pixel_queue_ragged_frames(ctx, size, variant):
     """The pixel-queue programs on frames smaller than a block or with a
@@ -411,7 +427,7 @@ def test_fp32_hit_window_option(ctx, ctx64):
 # A cell scene whose program has bounds capacity 64, and the variants run on it.
 CAPACITY_BASE = lambda: scenes.hyperbolic_cells(64, 48, primitives=48)  # noqa: E731
 CAPACITY_VARIANTS = ["flat", "scalar_bounds", "sb_groups4", "groups4", "pixel_queue", "sb_g4_q_prio", "gq4_ho_prio",
-                     "gq4_ho_hl", "gq4_ho_hl_s32", "gq4_s64"]
+                     "gq4_ho_hl", "gq4_ho_hl_s32", "gq4_s64", "gq4_ho_q_hl_s16"]
 
 
 @pytest.mark.parametrize("variant", CAPACITY_VARIANTS)
