@@ -540,7 +540,7 @@ def spp_for(fit, target):
 
 
 def fixed_knobs(text, base=None):
-    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN]" (no timing)."""
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN]" (no timing)."""
     from hypertrace_amd.processing import STRICT_FP
     from hypertrace_amd.tuning import options
     knobs = text.split(",")
@@ -554,12 +554,13 @@ def fixed_knobs(text, base=None):
     ho = "ho" in flags
     hl = "hl" in flags
     sparse = next((int(k[1:]) for k in flags if k.startswith("s") and k[1:].isdigit()), 0)
+    spread = next((int(k[2:]) for k in flags if k.startswith("sp") and k[2:].isdigit()), 0)
     return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
                      "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho, "helpers": hl,
-                     "sparse": sparse,
+                     "sparse": sparse, "spread": spread,
                      "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups,
                                         pixel_queue=queue, wave_prio=prio, group_queue=gq, hit_order=ho,
-                                        helpers=hl, sparse=sparse)},
+                                        helpers=hl, sparse=sparse, spread=spread)},
             "table": [], "fixed": True}
 
 
@@ -781,10 +782,10 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
-    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl]",
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN]",
                     help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
                          "instead of tuning (keeps a profiled trace to one program)")
-    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl]",
+    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN]",
                     help="the same for the SDF leg's program")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "

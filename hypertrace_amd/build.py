@@ -153,7 +153,8 @@ def _suite_jobs():
         jobs += [(scenes.euclidean_spheres(w, h).scene, X32_F32, par.CULLED_VARIANTS[v], None)
                  for (w, h) in ((1, 1), (64, 48)) for v in ("pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio",
                                                            "gq4_ho_hl", "gq4_ho_q_s16", "gq4_ho_q_hl_s16",
-                                                           "q_hl")]
+                                                           "q_hl", "gq4_ho_q_hl_s32_sp64", "sp64", "q_hl_sp4",
+                                                           "gq4_ho_hl_s32_sp8_w3")]
         jobs += [(eu, X32_F32, program_options(eps32=1e-5), None),
                  (eu, X32_F32, program_options(eps32=1e-5, no_cull=True), None)]
     bt = load("test_gpu_batch")

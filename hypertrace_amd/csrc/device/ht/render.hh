@@ -344,6 +344,28 @@ template <class S> struct QueuedPixel {
 #endif
 };
 
+#ifdef HT_PIXEL_SPREAD
+// Program option HT_PIXEL_SPREAD = B (a power of two, 2..64): the pixel of
+// work item t of render_passes. The frame's pixels, padded to whole waves
+// (nw = ceil(n / 64) of them), are cut into B bands of nw * 64 / B pixels;
+// wave k takes the k-th run of 64 / B consecutive pixels of every band.
+// A cluster of expensive pixels (the scene's dense regions) is so spread
+// over many waves instead of filling a few, each of which would hold the
+// frame's end alone; the lanes of a wave whose own pixels finish early help
+// the scans of the rest (HT_HELPERS). A bijection of [0, nw * 64); items
+// that land on padding (>= n) take no pixel. Pixels are independent, so
+// canvas and seeds are those of the identity order bit for bit.
+static_assert(HT_PIXEL_SPREAD >= 2 && HT_PIXEL_SPREAD <= 64 && (HT_PIXEL_SPREAD & (HT_PIXEL_SPREAD - 1)) == 0,
+              "HT_PIXEL_SPREAD: a power of two from 2 to 64");
+HD uint spread_pixel(uint t, uint n) {
+    const uint run = 64u / (uint)HT_PIXEL_SPREAD;
+    const uint nw = (n + 63u) / 64u;
+    const uint wave = t / 64u, lane = t % 64u;
+    return ((lane / run) * nw + wave) * run + lane % run;
+}
+HD uint spread_extent(uint n) { return (n + 63u) / 64u * 64u; }
+#endif
+
 #ifdef HT_PIXEL_QUEUE
 // Program option HT_PIXEL_QUEUE: render_passes as a persistent kernel (the
 // runtime launches only the blocks that fit on the device at once, after
@@ -360,6 +382,11 @@ template <class S>
 HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
                              uint *__restrict__ seeds, uint passes) {
     const uint n = w * h;
+#ifdef HT_PIXEL_SPREAD
+    const uint extent = spread_extent(n);  // the counter runs over the spread order's items
+#else
+    const uint extent = n;
+#endif
     const unsigned long long below = lane_mask_below();
     QueuedPixel<S> p;
     bool drained = false;
@@ -394,8 +421,11 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
                 uint base = 0;
                 if (__lane_id() == leader) base = atomicAdd(&ht_pixel_next, (uint)__popcll(m));
                 base = __shfl(base, (int)leader);
-                if (base + (uint)__popcll(m) >= n) drained = true;  // the counter only grows
+                if (base + (uint)__popcll(m) >= extent) drained = true;  // the counter only grows
                 uint cand = base + (uint)__popcll(m & below);
+#ifdef HT_PIXEL_SPREAD
+                if (need && cand < extent) cand = spread_pixel(cand, n);
+#endif
                 if (need && cand < n) p.take(cand, w, h, scene, canvas, seeds, passes, S::HOPS);
             }
         }
@@ -421,6 +451,10 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
     render_passes_queued<S>(w, h, scene, canvas, seeds, passes);
 #else
     uint idx = blockIdx.x * blockDim.x + threadIdx.x;
+#ifdef HT_PIXEL_SPREAD
+    if (idx >= spread_extent(w * h)) return;
+    idx = spread_pixel(idx, w * h);
+#endif
     if (idx >= w * h) return;
     render_pixel_passes<S>(w, h, idx, scene, canvas, seeds, passes, S::HOPS);
 #endif

@@ -34,7 +34,12 @@ once the queue is empty) in the loop as helpers that take other rays' pair
 tests, so a wave's last rays are not tested on nearly empty lanes;
 HT_SPARSE_L1 = R (with the group queue) runs the group tests of a scan
 with at most R rays over its (group, ray) pairs, one pair per lane, instead
-of every group in every lane. Every
+of every group in every lane; HT_PIXEL_SPREAD = B hands the pixels to the
+lanes (or out of the pixel queue) in an order that gives each wave one run
+of 64 / B pixels from each of B bands of the frame, so that a cluster of
+expensive pixels is spread over many waves whose other lanes finish early
+and help (with the pixel queue and B = 64 the 640x480 cell frame takes 17
+instead of 21 ms). Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -67,12 +72,14 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 4, False, True, True, True, True, 32), (4, 4, True, 4, False, True, True, True, True, 16),
     (4, 3, True, 4, False, True, True, True, True, 32),
     (4, 4, True, 4, True, True, True, True, True), (4, 4, True, 4, True, True, True, True, True, 16),
-    (4, 4, True, 4, True, True, True, False, True), (4, 4, True, 4, True, True, True, False, True, 16))
+    (4, 4, True, 4, True, True, True, False, True), (4, 4, True, 4, True, True, True, False, True, 16),
+    (4, 4, True, 4, True, True, True, True, True, 32, 64), (4, 4, True, 4, True, True, True, True, True, 16, 64),
+    (4, 3, True, 4, True, True, True, True, True, 16, 64), (4, 3, True, 4, False, True, True, True, True, 32, 8))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
             pixel_queue: bool = False, wave_prio: bool = False, group_queue: bool = False,
-            hit_order: bool = False, helpers: bool = False, sparse: int = 0) -> str:
+            hit_order: bool = False, helpers: bool = False, sparse: int = 0, spread: int = 0) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
@@ -90,10 +97,12 @@ def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool 
         opts += " -DHT_HELPERS=1"
     if sparse:
         opts += f" -DHT_SPARSE_L1={int(sparse)}"
+    if spread:
+        opts += f" -DHT_PIXEL_SPREAD={int(spread)}"
     return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool, bool, int]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool, bool, int, int]:
     unroll, waves = int(candidate[0]), int(candidate[1])
     sb = bool(candidate[2]) if len(candidate) > 2 else False
     groups = int(candidate[3]) if len(candidate) > 3 else 0
@@ -103,7 +112,8 @@ def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, 
     ho = bool(candidate[7]) if len(candidate) > 7 else False
     hl = bool(candidate[8]) if len(candidate) > 8 else False
     sparse = int(candidate[9]) if len(candidate) > 9 else 0
-    return unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse
+    spread = int(candidate[10]) if len(candidate) > 10 else 0
+    return unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse, spread
 
 
 def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
@@ -149,8 +159,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse = _knobs(cand)
-        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho, hl, sparse)
+        unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse, spread = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho, hl, sparse, spread)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -163,7 +173,7 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
         res = r.program.kernel_info("render_passes")
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
                       "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho, "helpers": hl,
-                      "sparse": sparse, "ms": sorted(times)[len(times) // 2], "options": opts,
+                      "sparse": sparse, "spread": spread, "ms": sorted(times)[len(times) // 2], "options": opts,
                       "vgprs": res["vgprs"], "scratch": res["scratch"]})
     best = pick(table, spill_tolerance)
     result = {"best": best, "table": table, "fastest_any": min(table, key=lambda t: t["ms"]),

@@ -262,6 +262,17 @@ CULLED_VARIANTS = {
     "gq4_q_hl": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_PIXEL_QUEUE=1"
                 " -DHT_HELPERS=1",
     "q_hl": STRICT_FP + " -DHT_PIXEL_QUEUE=1 -DHT_HELPERS=1",
+    # pixels handed out in the spread order (HT_PIXEL_SPREAD): one run of
+    # 64 / B pixels from each of B bands per wave, statically or from the
+    # pixel queue
+    "gq4_ho_q_hl_s32_sp64": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1"
+                            " -DHT_PIXEL_QUEUE=1 -DHT_WAVE_PRIO=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1"
+                            " -DHT_SPARSE_L1=32 -DHT_PIXEL_SPREAD=64",
+    "gq4_ho_hl_s32_sp8_w3": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1"
+                            " -DHT_WAVE_PRIO=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1 -DHT_SPARSE_L1=32"
+                            " -DHT_PIXEL_SPREAD=8 -DHT_WAVES_PER_EU=3",
+    "q_hl_sp4": STRICT_FP + " -DHT_PIXEL_QUEUE=1 -DHT_HELPERS=1 -DHT_PIXEL_SPREAD=4",
+    "sp64": STRICT_FP + " -DHT_PIXEL_SPREAD=64",
 }
 
 
@@ -293,7 +304,9 @@ CULLED_VARIANTS = {
     ("sdf_1024", False, "gq4_s64"), ("sdf_1024", False, "gq4_ho_hl_s32"), ("eu_ball", False, "gq4_ho_q_s16"),
     ("hy_example", False, "gq4_ho_q_s16"), ("eu_example", False, "gq8_s8_u2"), ("hy_cells", False, "gq8_s8_u2"),
     ("hy_cells", True, "gq4_ho_q_hl_s16"), ("sdf_1024", False, "gq4_q_hl"), ("eu_ball", False, "gq4_ho_q_hl_s16"),
-    ("hy_example", False, "gq4_q_hl"), ("sdf", False, "q_hl"), ("eu_example", False, "q_hl")])
+    ("hy_example", False, "gq4_q_hl"), ("sdf", False, "q_hl"), ("eu_example", False, "q_hl"),
+    ("hy_cells", True, "gq4_ho_q_hl_s32_sp64"), ("hy_cells", False, "gq4_ho_hl_s32_sp8_w3"),
+    ("sdf_1024", False, "gq4_ho_q_hl_s32_sp64"), ("eu_ball", False, "q_hl_sp4"), ("hy_example", False, "sp64")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
@@ -338,7 +351,7 @@ def test_helpers_with_russian_roulette(ctx, name, helpers):
 
 
 @pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3", "gq4_ho_hl", "gq4_ho_hl_s32", "gq4_ho_q_s16",
-                                     "gq4_ho_q_hl_s16"])
+                                     "gq4_ho_q_hl_s16", "gq4_ho_q_hl_s32_sp64"])
 @pytest.mark.parametrize("frame", ["640x480", "1920x1080"])
 def test_hit_order_identical_on_full_frames(ctx, frame, variant):
     """The hit-order scan (each passing pair's crossing interval: pairs that
@@ -364,12 +377,14 @@ def test_hit_order_identical_on_full_frames(ctx, frame, variant):
 
 
 @pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio", "gq4_ho_hl",
-                                     "gq4_ho_q_s16", "gq4_ho_q_hl_s16", "q_hl"])
+                                     "gq4_ho_q_s16", "gq4_ho_q_hl_s16", "q_hl", "gq4_ho_q_hl_s32_sp64", "sp64",
+                                     "q_hl_sp4", "gq4_ho_hl_s32_sp8_w3"])
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
 def test_pixel_queue_ragged_frames(ctx, size, variant):
-    """The pixel-queue programs on frames smaller than a block or with a
-    partial last wave, launched twice (the counter is reset before every
-    launch): the same canvas and seeds as the one-thread-per-pixel kernel."""
+    """The pixel-queue and spread-order programs on frames smaller than a
+    block or with a partial last wave (the spread order's padding items take
+    no pixel), launched twice (the counter is reset before every launch):
+    the same canvas and seeds as the one-thread-per-pixel kernel."""
     w, h = size
     wl = scenes.euclidean_spheres(w, h)
     print(f"case {variant} {w}x{h}", flush=True)
