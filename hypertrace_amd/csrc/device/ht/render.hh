@@ -345,25 +345,56 @@ template <class S> struct QueuedPixel {
 };
 
 #ifdef HT_PIXEL_SPREAD
-// Program option HT_PIXEL_SPREAD = B (a power of two, 2..64): the pixel of
-// work item t of render_passes. The frame's pixels, padded to whole waves
-// (nw = ceil(n / 64) of them), are cut into B bands of nw * 64 / B pixels;
-// wave k takes the k-th run of 64 / B consecutive pixels of every band.
-// A cluster of expensive pixels (the scene's dense regions) is so spread
-// over many waves instead of filling a few, each of which would hold the
-// frame's end alone; the lanes of a wave whose own pixels finish early help
-// the scans of the rest (HT_HELPERS). A bijection of [0, nw * 64); items
-// that land on padding (>= n) take no pixel. Pixels are independent, so
-// canvas and seeds are those of the identity order bit for bit.
-static_assert(HT_PIXEL_SPREAD >= 2 && HT_PIXEL_SPREAD <= 64 && (HT_PIXEL_SPREAD & (HT_PIXEL_SPREAD - 1)) == 0,
-              "HT_PIXEL_SPREAD: a power of two from 2 to 64");
-HD uint spread_pixel(uint t, uint n) {
-    const uint run = 64u / (uint)HT_PIXEL_SPREAD;
-    const uint nw = (n + 63u) / 64u;
-    const uint wave = t / 64u, lane = t % 64u;
-    return ((lane / run) * nw + wave) * run + lane % run;
-}
-HD uint spread_extent(uint n) { return (n + 63u) / 64u * 64u; }
+// Program option HT_PIXEL_SPREAD: the order in which render_passes hands
+// the frame's n pixels to its lanes (statically, or out of the pixel
+// queue). Work item t < extent takes pixel(t); items with pixel(t) >= n
+// take none. Pixels are independent, so canvas and seeds are those of the
+// identity order bit for bit; only which pixels share a wave changes. A
+// cluster of expensive pixels (the scene's dense regions) is so spread over
+// many waves instead of filling a few, each of which would hold the
+// frame's end alone, and the lanes of a wave whose own pixels finish early
+// help the scans of the rest (HT_HELPERS).
+//   B = 2..64 (a power of two): bands. The pixels, padded to whole waves
+//     (nw = ceil(n / 64)), are cut into B bands of nw * 64 / B; wave k takes
+//     the k-th run of 64 / B consecutive pixels of every band (a bijection
+//     of [0, nw * 64)).
+//   1: golden-ratio order. pixel(t) = t * a mod n with a the first integer
+//     >= 0.618 n coprime to n (a bijection of [0, n)): consecutive items lie
+//     about 0.382 n apart, so a wave's 64 pixels sample the whole frame.
+static_assert(HT_PIXEL_SPREAD == 1 ||
+                  (HT_PIXEL_SPREAD >= 2 && HT_PIXEL_SPREAD <= 64 && (HT_PIXEL_SPREAD & (HT_PIXEL_SPREAD - 1)) == 0),
+              "HT_PIXEL_SPREAD: 1 (golden-ratio order) or a power of two from 2 to 64 (bands)");
+struct PixelOrder {
+    uint n, extent, step;
+    HD explicit PixelOrder(uint n_) : n(n_) {
+#if HT_PIXEL_SPREAD == 1
+        extent = n;
+        step = (uint)((unsigned long long)n * 2654435769ull >> 32);  // floor(n * 0.6180339887)
+        for (;; ++step) {
+            uint a = step, b = n;
+            while (b != 0u) {
+                uint r = a % b;
+                a = b;
+                b = r;
+            }
+            if (a == 1u) break;  // gcd(step, n) = 1; n = 1 stops at step 1 (gcd(0, 1) = 1 too)
+        }
+#else
+        extent = (n + 63u) / 64u * 64u;
+        step = 0u;
+#endif
+    }
+    HD uint pixel(uint t) const {
+#if HT_PIXEL_SPREAD == 1
+        return (uint)((unsigned long long)t * step % n);
+#else
+        const uint run = 64u / (uint)HT_PIXEL_SPREAD;
+        const uint nw = extent / 64u;
+        const uint wave = t / 64u, lane = t % 64u;
+        return ((lane / run) * nw + wave) * run + lane % run;
+#endif
+    }
+};
 #endif
 
 #ifdef HT_PIXEL_QUEUE
@@ -383,7 +414,8 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
                              uint *__restrict__ seeds, uint passes) {
     const uint n = w * h;
 #ifdef HT_PIXEL_SPREAD
-    const uint extent = spread_extent(n);  // the counter runs over the spread order's items
+    const PixelOrder order(n);
+    const uint extent = order.extent;  // the counter runs over the order's items
 #else
     const uint extent = n;
 #endif
@@ -424,7 +456,7 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
                 if (base + (uint)__popcll(m) >= extent) drained = true;  // the counter only grows
                 uint cand = base + (uint)__popcll(m & below);
 #ifdef HT_PIXEL_SPREAD
-                if (need && cand < extent) cand = spread_pixel(cand, n);
+                if (need && cand < extent) cand = order.pixel(cand);
 #endif
                 if (need && cand < n) p.take(cand, w, h, scene, canvas, seeds, passes, S::HOPS);
             }
@@ -452,8 +484,9 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 #else
     uint idx = blockIdx.x * blockDim.x + threadIdx.x;
 #ifdef HT_PIXEL_SPREAD
-    if (idx >= spread_extent(w * h)) return;
-    idx = spread_pixel(idx, w * h);
+    const PixelOrder order(w * h);
+    if (idx >= order.extent) return;
+    idx = order.pixel(idx);
 #endif
     if (idx >= w * h) return;
     render_pixel_passes<S>(w, h, idx, scene, canvas, seeds, passes, S::HOPS);

@@ -34,12 +34,13 @@ once the queue is empty) in the loop as helpers that take other rays' pair
 tests, so a wave's last rays are not tested on nearly empty lanes;
 HT_SPARSE_L1 = R (with the group queue) runs the group tests of a scan
 with at most R rays over its (group, ray) pairs, one pair per lane, instead
-of every group in every lane; HT_PIXEL_SPREAD = B hands the pixels to the
-lanes (or out of the pixel queue) in an order that gives each wave one run
-of 64 / B pixels from each of B bands of the frame, so that a cluster of
+of every group in every lane; HT_PIXEL_SPREAD hands the pixels to the
+lanes (or out of the pixel queue) in an order that spreads each wave's 64
+pixels over the frame (1: the golden-ratio order t * a mod n; B = 2..64:
+one run of 64 / B pixels from each of B bands), so that a cluster of
 expensive pixels is spread over many waves whose other lanes finish early
-and help (with the pixel queue and B = 64 the 640x480 cell frame takes 17
-instead of 21 ms). Every
+and help (with the pixel queue and the golden-ratio order the 640x480 cell
+frame takes 16.8 instead of 20.6 ms). Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -73,8 +74,8 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 3, True, 4, False, True, True, True, True, 32),
     (4, 4, True, 4, True, True, True, True, True), (4, 4, True, 4, True, True, True, True, True, 16),
     (4, 4, True, 4, True, True, True, False, True), (4, 4, True, 4, True, True, True, False, True, 16),
-    (4, 4, True, 4, True, True, True, True, True, 32, 64), (4, 4, True, 4, True, True, True, True, True, 16, 64),
-    (4, 3, True, 4, True, True, True, True, True, 16, 64), (4, 3, True, 4, False, True, True, True, True, 32, 8))
+    (4, 4, True, 4, True, True, True, True, True, 32, 1), (4, 4, True, 4, True, True, True, True, True, 16, 1),
+    (4, 3, True, 4, True, True, True, True, True, 32, 1), (4, 4, True, 4, True, True, True, True, True, 32, 64))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,

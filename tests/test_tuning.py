@@ -95,6 +95,7 @@ def test_bench_fixed_knobs_and_gpu_count_guard():
     k = bench.fixed_knobs("4,4,sb,g4,p,gq,ho,hl,s32")["best"]
     assert k["sparse"] == 32 and k["options"] == candidate_options((4, 4, True, 4, False, True, True, True, True, 32))
     assert k["options"].endswith("-DHT_HELPERS=1 -DHT_SPARSE_L1=32") and bench.count_mode(k) == 2004
+    assert bench.fixed_knobs("4,4,sb,g4,q,p,gq,ho,hl,s32,sp1")["best"]["options"].endswith("-DHT_PIXEL_SPREAD=1")
     k = bench.fixed_knobs("4,4,sb,g4,q,p,gq,ho,hl,s32,sp64")["best"]
     assert (k["sparse"], k["spread"], k["pixel_queue"]) == (32, 64, True)
     assert k["options"] == candidate_options((4, 4, True, 4, True, True, True, True, True, 32, 64))
