@@ -78,3 +78,37 @@ def test_pipeline_set_scene_then_render(ctx):
     assert not np.array_equal(ca.load_image(), c.load_image())
     assert rgba.shape == (H, W, 4) and np.all(rgba[..., 3] == 255)
     assert np.array_equal(p.get_image(fused=True), rgba)
+
+
+def test_interactive_loop_on_the_device(ctx):
+    """The headless interactive loop (hypertrace_amd/viewer.py) over the
+    reference's hyperbolic example: frames without input accumulate passes;
+    a held key moves the camera, which cleans the canvas and continues every
+    pixel's RNG stream: the frame then equals a fresh canvas seeded with the
+    seeds left by the earlier frames, rendered over the moved scene."""
+    from hypertrace_amd.viewer import Handler, Hyperbolic3, Input, IsotropicController, run
+
+    w, h = 48, 32
+    wl = scenes.reference_hyperbolic_example(w, h)
+    scene = wl.scene
+    pipe = Pipeline(ctx, (w, h), scene, seed_mode=wl.seed_mode, seed=wl.seed)
+    ctrl = IsotropicController(Hyperbolic3, scene.view.map, 1.0)
+    handler = Handler((w, h))
+    frames = []
+    assert run(pipe, scene, ctrl, handler, [Input(), Input()], frames.append, passes_per_frame=2) == 2
+    assert pipe.canvas.passes == 4
+    assert np.all(pipe.canvas.load_image()[..., 3] == 4.0)
+    seeds = pipe.canvas.load_seeds().copy()
+    start = scene.view.map
+    assert run(pipe, scene, ctrl, handler, [Input(keys=frozenset({"W"}))], frames.append, passes_per_frame=3) == 1
+    assert scene.view.map is not start and pipe.canvas.passes == 3
+    r = Render(ctx, scene)
+    c = Canvas(ctx, (w, h), wl.seed_mode, wl.seed)
+    c.store_seeds(seeds)
+    buf = EntityBuffer(ctx, scene)
+    for _ in range(3):
+        r.render(buf, c)
+    np.testing.assert_array_equal(pipe.canvas.load_image().view(np.uint32), c.load_image().view(np.uint32))
+    np.testing.assert_array_equal(pipe.canvas.load_seeds(), c.load_seeds())
+    assert len(frames) == 3 and frames[2].shape == (h, w, 4) and np.all(frames[2][..., 3] == 255)
+    assert not np.array_equal(frames[1], frames[2])
