@@ -148,8 +148,17 @@ WAVE_QUEUE_PRIO = WAVE_QUEUE + " -DHT_WAVE_PRIO=1"
 # The wave queue with each job's bounds scanned in two levels (groups of 4
 # Morton-ordered elements; the per-job trees are built by batch_bounds_fill).
 WAVE_QUEUE_GROUPS = WAVE_QUEUE_PRIO + " -DHT_CULL_GROUPS=4"
+# The wave queue with the scene programs' two-level scan compacted at both
+# levels (group queue; each job's leaves read from its own FP64 array), the
+# hit order, helper lanes (lanes without a pixel once the tile's pixels are
+# taken join the scans) and sparse level 1.
+WAVE_QUEUE_HIT_ORDER = WAVE_QUEUE_GROUPS + " -DHT_GROUP_QUEUE=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1 -DHT_SPARSE_L1=32"
+# Block tiles with the same scan (static helper lanes).
+BLOCK_HIT_ORDER = SCALAR_BOUNDS_PRIO + " -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1" \
+                                       " -DHT_SPARSE_L1=32"
 OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (128, 256, 512, 1024)), (WAVE_QUEUE, (64, 128, 256)),
-           (WAVE_QUEUE_PRIO, (64, 128, 256)), (WAVE_QUEUE_GROUPS, (64, 128, 256)), (SCALAR_BOUNDS_PRIO, (128, 256)))
+           (WAVE_QUEUE_PRIO, (64, 128, 256)), (WAVE_QUEUE_GROUPS, (64, 128, 256)), (SCALAR_BOUNDS_PRIO, (128, 256)),
+           (WAVE_QUEUE_HIT_ORDER, (64, 128, 256)), (BLOCK_HIT_ORDER, (128, 256)))
 
 
 def tile_unit(opts: Optional[str], block: int) -> int:

@@ -136,7 +136,13 @@ HD void batch_wave_tile(const BatchJob &job, uint first, uint end, bool tail) {
             if (cursor >= end) break;
             continue;
         }
+#ifdef HT_HELPERS
+        // lanes without a pixel (the tile's pixels are all taken) help the
+        // scans of the wave's remaining rays
+        p.step_all(w, h, scene, canvas, seeds, passes, hops);
+#else
         if (p.have) p.step(w, h, scene, canvas, seeds, passes, hops);
+#endif
     }
 #ifdef HT_WAVE_PRIO
     if (level != 0) __builtin_amdgcn_s_setprio(0);

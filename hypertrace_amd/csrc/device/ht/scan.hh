@@ -86,9 +86,6 @@ HD uint gring_wrap(uint i) { return i >= GROUP_RING ? i - GROUP_RING : i; }
 #if defined(HT_GROUP_QUEUE) && !(defined(HT_SCALAR_BOUNDS) && defined(HT_CULL_GROUPS))
 #error "HT_GROUP_QUEUE needs HT_SCALAR_BOUNDS and HT_CULL_GROUPS"
 #endif
-#if defined(HT_GROUP_QUEUE) && defined(HT_BATCH_BLOCK)
-#error "HT_GROUP_QUEUE is an option of scene programs"
-#endif
 #if defined(HT_SPARSE_L1) && !defined(HT_GROUP_QUEUE)
 #error "HT_SPARSE_L1 is an option of the group queue (HT_GROUP_QUEUE)"
 #endif
@@ -206,6 +203,13 @@ HD const ConstBoundS *scalar_bounds_base() {
     uint hi = __builtin_amdgcn_readfirstlane((uint)(v >> 32));
     return (const ConstBoundS *)(((unsigned long long)hi << 32) | lo);
 }
+// The same array for loads at per-lane indices (vector loads).
+HD const BoundS *vector_bounds_base() {
+    unsigned long long v = batch_bounds_slot();
+    uint lo = __builtin_amdgcn_readfirstlane((uint)v);
+    uint hi = __builtin_amdgcn_readfirstlane((uint)(v >> 32));
+    return (const BoundS *)(((unsigned long long)hi << 32) | lo);
+}
 #else
 // Scene programs: the top-level vector's bounds, written by the program's
 // bounds_fill kernel before each render launch (launches of one program on
@@ -218,6 +222,7 @@ __device__ BoundS ht_scalar_bounds[HT_BOUNDS_CAP + HT_BOUNDS_CAP / HT_CULL_GROUP
 __device__ BoundS ht_scalar_bounds[HT_BOUNDS_CAP];
 #endif
 HD const ConstBoundS *scalar_bounds_base() { return (const ConstBoundS *)ht_scalar_bounds; }
+HD const BoundS *vector_bounds_base() { return ht_scalar_bounds; }
 #endif
 // One 64-byte scalar load per bound (s_load_dwordx16), whatever fields the
 // test uses.
@@ -692,10 +697,28 @@ template <class Tag> HD Bound *bounds_lds() {
     __shared__ Bound b[BOUNDS_CAP];
     return b;
 }
-template <class G> HD WaveScratch<G> *wave_scratch_lds() {
-    __shared__ WaveScratch<G> s[MAX_WAVES];
-    return s;
+#ifdef HT_BATCH_BLOCK
+// Batch programs inline both geometries' scans, but a wave scans one job at
+// a time and every scan rewrites the scratch it reads: the two geometries
+// share one LDS buffer.
+static constexpr size_t WAVE_SCRATCH_BYTES =
+    ((sizeof(WaveScratch<Eu>) > sizeof(WaveScratch<Hy>) ? sizeof(WaveScratch<Eu>) : sizeof(WaveScratch<Hy>)) + 15) /
+    16 * 16;
+HD unsigned char *wave_scratch_raw() {
+    __shared__ __attribute__((aligned(16))) unsigned char raw[MAX_WAVES * WAVE_SCRATCH_BYTES];
+    return raw;
 }
+// Wave w's slot, WAVE_SCRATCH_BYTES apart whatever the geometry.
+template <class G> HD WaveScratch<G> *wave_scratch_lds(uint w) {
+    static_assert(alignof(WaveScratch<G>) <= 16 && WAVE_SCRATCH_BYTES % 16 == 0, "scratch alignment");
+    return reinterpret_cast<WaveScratch<G> *>(wave_scratch_raw() + w * WAVE_SCRATCH_BYTES);
+}
+#else
+template <class G> HD WaveScratch<G> *wave_scratch_lds(uint w) {
+    __shared__ WaveScratch<G> s[MAX_WAVES];
+    return s + w;
+}
+#endif
 
 // ------------------------------------------------------------ bounding-sphere tree
 // Implicit complete binary tree over the elements in Morton order of their
@@ -856,7 +879,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     // All threads of the block, before any of them diverges; the caller
     // synchronises the block afterwards.
     static HD void prepare(const u8 *self) {
-#if defined(HT_GROUP_QUEUE)
+#if defined(HT_GROUP_QUEUE) && !defined(HT_BATCH_BLOCK)
         // the tree's leaves (Morton order, element index in pad1) from the
         // global array bounds_fill wrote, narrowed to FP32: their values are
         // FP32 numbers widened (the squares FP64-exact), so the narrowed
@@ -1071,7 +1094,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             uint e = ws->queue[ring_wrap(head + arank)];
             uint i = HIT_ELEM(e), L = e & 63u;
 #ifdef HT_HIT_ORDER
-            if (slots) i = __float_as_uint(bounds_lds<OV>()[i].pad1);  // the entry names the tree's leaf
+            if (slots) i = leaf_element_at(i);  // the entry names the tree's leaf
 #else
             (void)slots;
 #endif
@@ -1129,6 +1152,20 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
     }
 
 #if defined(HT_GROUP_QUEUE)
+#ifdef HT_BATCH_BLOCK
+    // Leaf slot i of the tree: batch programs read it from the job's global
+    // array (the waves of a block serve different jobs, so there is no
+    // per-block LDS copy) with vector loads; the FP64 values bounds_fill
+    // wrote decide as their FP32 narrowing in the scene programs' LDS copy.
+    static HD BoundS group_leaf(uint i) { return vector_bounds_base()[i]; }
+    static HD uint leaf_element(const BoundS &b) { return __float_as_uint(b.pad); }
+    static HD uint leaf_element_at(uint i) { return __float_as_uint(vector_bounds_base()[i].pad); }
+#else
+    static HD const Bound &group_leaf(uint i) { return bounds_lds<OV>()[i]; }
+    static HD uint leaf_element(const Bound &b) { return __float_as_uint(b.pad1); }
+    static HD uint leaf_element_at(uint i) { return __float_as_uint(bounds_lds<OV>()[i].pad1); }
+#endif
+
     // The two-level scan compacted at both levels (HT_GROUP_QUEUE).
     // push_owned(element, candidate, ray lane) appends to the exact tests'
     // ring, drain() runs its full batches.
@@ -1138,7 +1175,6 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                                       Drain &drain) {
         constexpr uint P = BOUNDS_CAP, GS = CULL_GROUP, FIRST = P / GS;
         static_assert(FIRST % CULL_UNROLL == 0, "level-1 steps cover the group nodes");
-        const Bound *leaves = bounds_lds<OV>();
         uint gcount = 0, ghead = 0;
         auto gpush = [&](uint k, bool pass, uint owner) {
             unsigned long long m = __ballot(pass);
@@ -1164,8 +1200,8 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 bool cand = false;
                 uint el = 0;
                 if (have) {
-                    const Bound &lb = leaves[gk * GS + j];
-                    el = __float_as_uint(lb.pad1);
+                    const auto &lb = group_leaf(gk * GS + j);
+                    el = leaf_element(lb);
                     cand = el != NO_ELEMENT && c2.candidate(lb);
                 }
                 // with HT_HIT_ORDER the entry names the leaf (its interval is
@@ -1194,7 +1230,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                 wave_sync();
                 const uint qs = nact / R, rs = nact % R;
                 uint r = arank % R, g = arank / R;  // pair base + arank = g * R + r
-                const BoundS *nodes = ht_scalar_bounds + P;
+                const BoundS *nodes = vector_bounds_base() + P;
                 for (uint base = 0; base < groups * R; base += nact) {
                     const uint L = ws->rlane[r];
                     bool pass = false;
@@ -1311,7 +1347,7 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
             const Bound *lb = bounds_lds<OV>();
             auto bound = [&](usize e) -> const Bound & { return lb[e]; };
 #endif
-            WaveScratch<G> *ws = wave_scratch_lds<G>() + (threadIdx.x / WAVE);
+            WaveScratch<G> *ws = wave_scratch_lds<G>(threadIdx.x / WAVE);
             uint lane = __lane_id();
             uint orig = ctx->hasher.state;
             ws->ray[lane] = light->ray;
@@ -1357,7 +1393,6 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                     // the intervals of the pairs pushed since the last filter,
                     // one pair per active lane; a certain pair lowers its
                     // ray's bound, a dropped one gets lb = +inf
-                    const Bound *leaves = bounds_lds<OV>();
                     for (uint base = 0; base < qcount; base += nact) {
                         uint pos = base + arank;
                         if (pos < qcount) {
@@ -1365,9 +1400,9 @@ template <class G, class E> struct TopScan<ObjectVec<G, E>> {
                             uint e = ws->queue[at];
                             if ((e >> 16) == HIT_LB_PENDING) {
                                 uint L = e & 63u;
-                                const Bound &b = leaves[HIT_ELEM(e)];
+                                const auto &b = group_leaf(HIT_ELEM(e));
                                 Hasher hh = {ws->orig_hasher[L]};
-                                hash_usize(&hh, __float_as_uint(b.pad1));
+                                hash_usize(&hh, leaf_element(b));
                                 bool exempt = hh.state == ws->prev_hash[L];  // the repeat rule decides its test
                                 HitIv iv = hit_order_interval(b, ws->ray[L], ws->ivc[L], exempt);
                                 if (iv.certain) atomicMin(&ws->tw[L], __float_as_uint(iv.ub));

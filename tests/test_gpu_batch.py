@@ -8,8 +8,8 @@ import pytest
 from hypertrace_amd import scenes
 import dataclasses
 
-from hypertrace_amd.batch import (SCALAR_BOUNDS, SCALAR_BOUNDS_PRIO, WAVE_QUEUE, WAVE_QUEUE_PRIO, Batch, BatchProgram,
-                                  autotune)
+from hypertrace_amd.batch import (BLOCK_HIT_ORDER, SCALAR_BOUNDS, SCALAR_BOUNDS_PRIO, WAVE_QUEUE, WAVE_QUEUE_HIT_ORDER,
+                                  WAVE_QUEUE_PRIO, Batch, BatchProgram, autotune)
 from hypertrace_amd.native import HtError
 from hypertrace_amd.objects import SceneImpl
 from hypertrace_amd.processing import Canvas, Context, EntityBuffer, Render
@@ -45,8 +45,12 @@ IDENTITY_CASES = [(256, 1, None), (64, 4, None), (1024, 1, None), (256, 1, SCALA
                   (128, 1, WAVE_QUEUE), (256, 4, WAVE_QUEUE), (64, 1, WAVE_QUEUE), (256, 0.25, WAVE_QUEUE),
                   (256, 1, SCALAR_BOUNDS + GROUPS4), (128, 0.5, WAVE_QUEUE + GROUPS4),
                   (64, 2, WAVE_QUEUE + " -DHT_CULL_GROUPS=8"), (128, 1, SCALAR_BOUNDS + " -DHT_WAVE_PRIO=1"),
-                  (128, 1, WAVE_QUEUE + " -DHT_WAVE_PRIO=1"), (64, 2, WAVE_QUEUE + GROUPS4 + " -DHT_WAVE_PRIO=1")]
-GRAPH_OPTS = [None, SCALAR_BOUNDS, WAVE_QUEUE, WAVE_QUEUE + GROUPS4]  # 256-thread programs
+                  (128, 1, WAVE_QUEUE + " -DHT_WAVE_PRIO=1"), (64, 2, WAVE_QUEUE + GROUPS4 + " -DHT_WAVE_PRIO=1"),
+                  # the group queue with the hit order (each job's leaves from its own array), helper lanes and
+                  # sparse level 1, over wave tiles and block tiles; the group queue alone
+                  (64, 1, WAVE_QUEUE_HIT_ORDER), (128, 2, WAVE_QUEUE_HIT_ORDER), (256, 0.25, WAVE_QUEUE_HIT_ORDER),
+                  (128, 1, BLOCK_HIT_ORDER), (256, 1, WAVE_QUEUE + GROUPS4 + " -DHT_GROUP_QUEUE=1")]
+GRAPH_OPTS = [None, SCALAR_BOUNDS, WAVE_QUEUE, WAVE_QUEUE + GROUPS4, WAVE_QUEUE_HIT_ORDER]  # 256-thread programs
 
 
 @pytest.mark.parametrize("block,mult,opts", IDENTITY_CASES)
@@ -119,7 +123,8 @@ def test_batch_jobs_with_zero_spp(ctx, opts):
 # program's capacity (256: the unculled scan), across the tree's 16-element
 # threshold, and up to the capacity; (primitive range, program options, block).
 EDGE_SETS = [((257, 300), None, 256), ((257, 300), WAVE_QUEUE + GROUPS4, 128), ((12, 20), WAVE_QUEUE + GROUPS4, 128),
-             ((250, 256), SCALAR_BOUNDS + GROUPS4, 256)]
+             ((250, 256), SCALAR_BOUNDS + GROUPS4, 256), ((12, 20), WAVE_QUEUE_HIT_ORDER, 128),
+             ((250, 256), WAVE_QUEUE_HIT_ORDER, 64), ((257, 300), WAVE_QUEUE_HIT_ORDER, 128)]
 
 
 def edge_specs(prims):
@@ -163,7 +168,7 @@ def test_batch_argument_checks(ctx):
 # ---- config 3 at its workload: 32 jobs, half Euclidean / half hyperbolic,
 # 128x128..1024x768, spp 1..64, depth 4..12, 8..256 primitives
 # The candidates the test's autotune picks from (the bench tunes over all).
-CFG3_OPTIONS = ((WAVE_QUEUE_PRIO, (128,)), (SCALAR_BOUNDS_PRIO, (256,)))
+CFG3_OPTIONS = ((WAVE_QUEUE_PRIO, (128,)), (SCALAR_BOUNDS_PRIO, (256,)), (WAVE_QUEUE_HIT_ORDER, (128,)))
 
 
 def test_config3_32_jobs_identical_to_single_renders(ctx):
@@ -176,20 +181,27 @@ def test_config3_32_jobs_identical_to_single_renders(ctx):
                     options=CFG3_OPTIONS)
     best = tune["best"]
     print(f"case config 3: {len(specs)} jobs, {tune['paths']} paths, best {best}", flush=True)
-    bprog = BatchProgram(ctx, best["block"], fp_options=best["options"])
-    batch = Batch(ctx, bprog, specs, best["tile"])
-    batch.render(best["grid"], clear=True)
+    batches = []
+    # the tuner's pick, and the hit-order wave-queue program whatever it picks
+    for block, opts, tile, grid in ((best["block"], best["options"], best["tile"], best["grid"]),
+                                    (128, WAVE_QUEUE_HIT_ORDER, 128, 2048)):
+        bprog = BatchProgram(ctx, block, fp_options=opts)
+        batch = Batch(ctx, bprog, specs, tile)
+        batch.render(grid, clear=True)
+        batches.append(batch)
     ctx.sync()
     for k, spec in enumerate(specs):
         img, seeds = alone(ctx, spec)
-        got = batch.canvases[k]
-        np.testing.assert_array_equal(got.load_image().view(np.uint32), img.view(np.uint32), err_msg=f"job {k}")
-        np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
         assert np.all(img[..., 3] == spec.spp)
+        for batch in batches:
+            got = batch.canvases[k]
+            np.testing.assert_array_equal(got.load_image().view(np.uint32), img.view(np.uint32), err_msg=f"job {k}")
+            np.testing.assert_array_equal(got.load_seeds(), seeds, err_msg=f"job {k}")
 
 
+@pytest.mark.parametrize("opts", [WAVE_QUEUE_PRIO, WAVE_QUEUE_HIT_ORDER])
 @pytest.mark.parametrize("geo_type", [0, 1])
-def test_config3_one_pass_crop_vs_oracle(ctx, geo_type):
+def test_config3_one_pass_crop_vs_oracle(ctx, geo_type, opts):
     """One pass of the first Euclidean (0) or hyperbolic (1) job of the
     32-job set, rendered by a batch program, against the CPU oracle on a
     crop: 1e-4 relative on 99.9 % of pixels, seeds equal."""
@@ -197,8 +209,8 @@ def test_config3_one_pass_crop_vs_oracle(ctx, geo_type):
 
     spec = next(s for s in scenes.batch_jobs() if s.type == geo_type)
     one = dataclasses.replace(spec, spp=1)
-    print(f"case config 3 crop: type {geo_type}, {one.width}x{one.height}, depth {one.depth}", flush=True)
-    bprog = BatchProgram(ctx, 128, fp_options=WAVE_QUEUE_PRIO)
+    print(f"case config 3 crop: type {geo_type}, {one.width}x{one.height}, depth {one.depth}, {opts}", flush=True)
+    bprog = BatchProgram(ctx, 128, fp_options=opts)
     batch = Batch(ctx, bprog, [one], 128)
     seeds0 = batch.canvases[0].load_seeds().copy()
     batch.render(grid=64, clear=True)
