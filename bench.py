@@ -188,7 +188,7 @@ def summary(out, detail_path=None):
     bt = out.get("batch_32_jobs")
     if bt:
         legs["batch_32_jobs"] = (
-            {"graph": bt["graph_replay"]["value"], "eager": bt["eager"]["value"],
+            {"graph": (bt.get("graph_replay") or {}).get("value"), "eager": bt["eager"]["value"],
              "one_job_per_launch": bt["one_job_per_launch"]["value"],
              "bit_identical": bt.get("bit_identical_to_single_jobs"),
              "small_jobs_graph_speedup": (bt.get("small_jobs") or {}).get("graph_speedup")}
@@ -692,8 +692,15 @@ def batch_leg(args):
     specs = scenes.batch_jobs()
     paths = sum(s.width * s.height * s.spp for s in specs)
     with Context(0, X32_F32) as ctx:
-        tune = autotune(ctx, specs, reps=1, progress=lambda m: print(m, file=sys.stderr, flush=True))
-        best = tune["best"]
+        if args.batch_program:
+            # a fixed program (NAME,BLOCK,TILE,GRID; NAME a program option set of batch.py), no tuning
+            from hypertrace_amd import batch as batch_mod
+            name, block, tile, grid_ = args.batch_program.split(",")
+            best = {"block": int(block), "tile": int(tile), "grid": int(grid_), "options": getattr(batch_mod, name)}
+            tune = {"best": best, "table": [dict(best, ms=0.0)], "fixed": True}
+        else:
+            tune = autotune(ctx, specs, reps=1, progress=lambda m: print(m, file=sys.stderr, flush=True))
+            best = tune["best"]
         bprog = BatchProgram(ctx, best["block"], fp_options=best.get("options"))
         batch = Batch(ctx, bprog, specs, best["tile"], with_rgba8=True)
         grid = best["grid"]
@@ -708,12 +715,17 @@ def batch_leg(args):
             return (time.perf_counter() - t0) / n
 
         eager_s = timed(lambda: batch.render(grid, clear=True, post=True))
-        batch.capture(grid, clear=True, post=True)
-        graph_s = timed(batch.replay)
+        graph_s = None
+        if not args.no_graph:
+            batch.capture(grid, clear=True, post=True)
+            graph_s = timed(batch.replay)
         # one job per launch with the best block-tiled candidate (a wave-queue
         # program's per-wave tiles leave most of the device idle on one job)
         solo = min((r for r in tune["table"] if "HT_BATCH_WAVE_QUEUE" not in (r.get("options") or "")),
-                   key=lambda r: r["ms"])
+                   key=lambda r: r["ms"], default=None)
+        if solo is None:  # a fixed wave-queue program: block tiles of the same scan for the solo launches
+            from hypertrace_amd.batch import BLOCK_HIT_ORDER
+            solo = {"block": 128, "tile": 128, "grid": 2048, "options": BLOCK_HIT_ORDER}
         sprog = BatchProgram(ctx, solo["block"], fp_options=solo.get("options"))
         singles = [Batch(ctx, sprog, [s], solo["tile"]) for s in specs]
 
@@ -746,20 +758,23 @@ def batch_leg(args):
         small_paths = sum(s.width * s.height * s.spp for s in small)
         sbatch = Batch(ctx, bprog, small, best["tile"], with_rgba8=True)
         small_eager_s = timed(lambda: sbatch.render(grid, clear=True, post=True), 20)
-        sbatch.capture(grid, clear=True, post=True)
-        small_graph_s = timed(sbatch.replay, 20)
+        small_graph_s = None
+        if not args.no_graph:
+            sbatch.capture(grid, clear=True, post=True)
+            small_graph_s = timed(sbatch.replay, 20)
         sbatch.close()
     small_out = {"workload": "256 jobs, 16x16..32x32, 1 spp, depth 4..6, 8..16 primitives, post pass per job",
                  "paths_per_frame": small_paths,
                  "eager": {"ms_per_frame": small_eager_s * 1e3, "value": small_paths / small_eager_s / 1e6,
                            "unit": "Mpaths/s"},
-                 "graph_replay": {"ms_per_frame": small_graph_s * 1e3, "value": small_paths / small_graph_s / 1e6,
-                                  "unit": "Mpaths/s"},
-                 "graph_speedup": small_eager_s / small_graph_s}
+                 "graph_replay": None if small_graph_s is None else
+                 {"ms_per_frame": small_graph_s * 1e3, "value": small_paths / small_graph_s / 1e6, "unit": "Mpaths/s"},
+                 "graph_speedup": None if small_graph_s is None else small_eager_s / small_graph_s}
     return {"workload": "32 jobs, half Euclidean / half hyperbolic, 128x128..1024x768, spp 1..64, depth 4..12, "
                         "8..256 primitives, diffuse/mirror/glass 0.6/0.25/0.15",
             "paths_per_frame": paths, "autotuned": best,
-            "graph_replay": {"ms_per_frame": graph_s * 1e3, "value": paths / graph_s / 1e6, "unit": "Mpaths/s"},
+            "graph_replay": None if graph_s is None else
+            {"ms_per_frame": graph_s * 1e3, "value": paths / graph_s / 1e6, "unit": "Mpaths/s"},
             "eager": {"ms_per_frame": eager_s * 1e3, "value": paths / eager_s / 1e6, "unit": "Mpaths/s"},
             "one_job_per_launch": {"ms_per_frame": single_s * 1e3, "value": paths / single_s / 1e6,
                                    "unit": "Mpaths/s", "program": solo},
@@ -783,6 +798,11 @@ def main():
     ap.add_argument("--no-euclidean", action="store_true", help="skip the 320x240 Euclidean leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-job batched leg")
     ap.add_argument("--no-wavefront", action="store_true", help="skip the wavefront-form leg")
+    ap.add_argument("--batch-program", default=None, metavar="NAME,BLOCK,TILE,GRID",
+                    help="batch leg: this program (NAME = an option set of hypertrace_amd/batch.py, e.g. "
+                         "WAVE_QUEUE_HIT_ORDER) instead of the autotuner's pick")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch leg: eager frames only, no hipGraph capture (for profilers that do not run graphs)")
     ap.add_argument("--wavefront-steps", type=int, default=2)
     ap.add_argument("--sdf-spp", type=int, default=256)
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
