@@ -377,7 +377,7 @@ struct PixelOrder {
                 a = b;
                 b = r;
             }
-            if (a == 1u) break;  // gcd(step, n) = 1; n = 1 stops at step 1 (gcd(0, 1) = 1 too)
+            if (a == 1u) break;  // gcd(step, n) = 1 (n = 1 keeps step 0: gcd(0, 1) = 1)
         }
 #else
         extent = (n + 63u) / 64u * 64u;

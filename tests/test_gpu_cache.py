@@ -203,7 +203,9 @@ def test_ablation_trained_cache(ctx):
     print(f"RMSE at {SPP} spp: plain {rmse_plain:.4f}, cache untrained {rmse_untrained:.4f}, "
           f"cache trained {rmse_trained:.4f}")
     assert rmse_trained < 0.8 * rmse_untrained
-    assert rmse_trained < 1.02 * rmse_plain
+    # what the trained network gains over the method without it: no worse
+    # than plain Monte Carlo at equal spp (measured 0.0321 against 0.0333)
+    assert rmse_trained < rmse_plain
 
 
 # ---- config 5 at its workload: the cell scene at 1920x1080 with 256 primitives
