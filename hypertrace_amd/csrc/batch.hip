@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "internal.hh"
@@ -113,15 +114,29 @@ extern "C" int ht_batch_create(ht_prog *prog, uint32_t njobs, const ht_batch_job
         return fail(HT_ERR_OTHER, std::string("ht_batch_create: tile must be a positive multiple of ") +
                                       (prog->batch_wave_tiles ? "64 (a wave-queue program)"
                                                               : "the program's block (" + std::to_string(block) + ")"));
-    std::vector<DevJob> table(njobs);
-    uint32_t tiles = 0;
     for (uint32_t i = 0; i < njobs; ++i) {
         int rc = batch_check_job(jobs[i], i);
         if (rc) return rc;
         if (jobs[i].scene->ctx != prog->ctx || jobs[i].canvas->ctx != prog->ctx || jobs[i].seeds->ctx != prog->ctx)
             return fail(HT_ERR_OTHER, "ht_batch_create: buffers on another context");
+    }
+    // Tiles are numbered job after job with the jobs in descending order of
+    // samples x depth per pixel (a bound on a pixel's work), so the longest
+    // tiles are taken from the counter first and the frame does not end on
+    // a heavy job's late tiles (longest processing time first). The device
+    // table is in that order; each job's pixels are rendered as before, so
+    // the order changes no result.
+    std::vector<uint32_t> order(njobs);
+    for (uint32_t i = 0; i < njobs; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t c) {
+        return (uint64_t)jobs[a].spp * jobs[a].depth > (uint64_t)jobs[c].spp * jobs[c].depth;
+    });
+    std::vector<DevJob> table(njobs);
+    uint32_t tiles = 0;
+    for (uint32_t k = 0; k < njobs; ++k) {
+        const uint32_t i = order[k];
         uint32_t px = jobs[i].w * jobs[i].h;
-        DevJob &d = table[i];
+        DevJob &d = table[k];
         std::memset(&d, 0, sizeof d);
         d.scene = (uint64_t)(uintptr_t)jobs[i].scene->ptr;
         d.canvas = (uint64_t)(uintptr_t)jobs[i].canvas->ptr;
