@@ -156,9 +156,14 @@ WAVE_QUEUE_HIT_ORDER = WAVE_QUEUE_GROUPS + " -DHT_GROUP_QUEUE=1 -DHT_HIT_ORDER=1
 # Block tiles with the same scan (static helper lanes).
 BLOCK_HIT_ORDER = SCALAR_BOUNDS_PRIO + " -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1" \
                                        " -DHT_SPARSE_L1=32"
+# The same two at a register cap of 3 waves per SIMD (170 VGPRs), where
+# the 4-wave cap (128) spills.
+WAVE_QUEUE_HIT_ORDER_W3 = WAVE_QUEUE_HIT_ORDER + " -DHT_WAVES_PER_EU=3"
+BLOCK_HIT_ORDER_W3 = BLOCK_HIT_ORDER + " -DHT_WAVES_PER_EU=3"
 OPTIONS = ((None, BLOCKS), (SCALAR_BOUNDS, (128, 256, 512, 1024)), (WAVE_QUEUE, (64, 128, 256)),
            (WAVE_QUEUE_PRIO, (64, 128, 256)), (WAVE_QUEUE_GROUPS, (64, 128, 256)), (SCALAR_BOUNDS_PRIO, (128, 256)),
-           (WAVE_QUEUE_HIT_ORDER, (64, 128, 256)), (BLOCK_HIT_ORDER, (128, 256)))
+           (WAVE_QUEUE_HIT_ORDER, (64, 128, 256)), (BLOCK_HIT_ORDER, (128, 256)),
+           (WAVE_QUEUE_HIT_ORDER_W3, (64, 128, 256)), (BLOCK_HIT_ORDER_W3, (128, 256)))
 
 
 def tile_unit(opts: Optional[str], block: int) -> int:
@@ -185,6 +190,7 @@ def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mu
     candidates = [(opts, block) for opts, opt_blocks in options for block in blocks if block in opt_blocks]
     for opts, block in candidates:
         bprog = BatchProgram(ctx, block, fp_options=opts)
+        res = bprog.program.kernel_info("render_batch")
         for m in tile_mults:
             tile = tile_unit(opts, block) * m
             batch = Batch(ctx, bprog, specs, tile)
@@ -198,10 +204,13 @@ def autotune(ctx: Context, specs: Sequence[BatchJobSpec], blocks=BLOCKS, tile_mu
                     batch.render(grid)
                     ctx.sync()
                     times.append(time.perf_counter() - t0)
-                table.append({"block": block, "tile": tile, "grid": grid, "options": opts, "ms": 1e3 * min(times)})
+                table.append({"block": block, "tile": tile, "grid": grid, "options": opts, "ms": 1e3 * min(times),
+                              "vgprs": res["vgprs"], "scratch": res["scratch"]})
             batch.close()
         if progress is not None:
-            progress(f"batch autotune: block {block} [{opts}]: best {min(r['ms'] for r in table[-len(tile_mults) * len(grid_waves_per_cu):]):.1f} ms")
+            progress(f"batch autotune: block {block} [{opts}]: best "
+                     f"{min(r['ms'] for r in table[-len(tile_mults) * len(grid_waves_per_cu):]):.1f} ms "
+                     f"({res['vgprs']} VGPRs, {res['scratch']} B scratch)")
     best = min(table, key=lambda r: r["ms"])
     result = {"best": best, "table": table, "paths": sum(s.width * s.height * s.spp for s in specs)}
     if cache_path:
