@@ -243,7 +243,9 @@ int ht_cache_train(ht_cache *cache, uint32_t steps, uint32_t batch, float lr);
  * culling bounds from its own FP64 array (allocated by ht_batch_create,
  * 64 B x the program's bounds capacity per job, filled at the start of every
  * frame by the program's batch_bounds_fill kernel) instead of staging them
- * in LDS per tile; same images. */
+ * in LDS per tile; same images. The frame's tiles are handed out job after
+ * job, the jobs in descending order of spp x depth (the heaviest tiles
+ * first); the order changes no result. */
 typedef struct ht_batch ht_batch;
 typedef struct ht_batch_job {
     uint32_t type;             /* 0 or 1: Scene0 or Scene1 of the batch program */
