@@ -546,7 +546,7 @@ def spp_for(fit, target):
 
 
 def fixed_knobs(text, base=None):
-    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN]" (no timing)."""
+    """A render-tuner result for the setting "UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN][,pc]" (no timing)."""
     from hypertrace_amd.processing import STRICT_FP
     from hypertrace_amd.tuning import options
     knobs = text.split(",")
@@ -561,12 +561,13 @@ def fixed_knobs(text, base=None):
     hl = "hl" in flags
     sparse = next((int(k[1:]) for k in flags if k.startswith("s") and k[1:].isdigit()), 0)
     spread = next((int(k[2:]) for k in flags if k.startswith("sp") and k[2:].isdigit()), 0)
+    classes = "pc" in flags
     return {"best": {"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
                      "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho, "helpers": hl,
-                     "sparse": sparse, "spread": spread,
+                     "sparse": sparse, "spread": spread, "classes": classes,
                      "options": options(unroll, waves, base or STRICT_FP, scalar_bounds=sb, groups=groups,
                                         pixel_queue=queue, wave_prio=prio, group_queue=gq, hit_order=ho,
-                                        helpers=hl, sparse=sparse, spread=spread)},
+                                        helpers=hl, sparse=sparse, spread=spread, classes=classes)},
             "table": [], "fixed": True}
 
 
@@ -808,10 +809,10 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--oracle-spp", type=int, default=4096)
     ap.add_argument("--target-rmse", type=float, default=0.01)
-    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN]",
+    ap.add_argument("--render-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN][,pc]",
                     help="use this cull unroll / waves-per-SIMD (/ scalar-bounds) setting for the main frame "
                          "instead of tuning (keeps a profiled trace to one program)")
-    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN]",
+    ap.add_argument("--sdf-knobs", default=None, metavar="UNROLL,WAVES[,sb][,gN][,q][,p][,gq][,ho][,hl][,sN][,spN][,pc]",
                     help="the same for the SDF leg's program")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run the extract kernel over 4096x4096 float4 (256 MiB read + 256 MiB written): "

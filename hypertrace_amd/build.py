@@ -155,7 +155,8 @@ def _suite_jobs():
                                                            "gq4_ho_hl", "gq4_ho_q_s16", "gq4_ho_q_hl_s16",
                                                            "q_hl", "gq4_ho_q_hl_s32_sp64", "sp64", "q_hl_sp4",
                                                            "gq4_ho_hl_s32_sp8_w3", "gq4_ho_q_hl_s32_sp1",
-                                                           "q_hl_sp1", "sp1_w3")]
+                                                           "q_hl_sp1", "sp1_w3", "gq4_ho_q_hl_s32_sp1_pc",
+                                                           "q_sp1_pc")]
         jobs += [(eu, X32_F32, program_options(eps32=1e-5), None),
                  (eu, X32_F32, program_options(eps32=1e-5, no_cull=True), None)]
     bt = load("test_gpu_batch")

@@ -397,6 +397,51 @@ struct PixelOrder {
 };
 #endif
 
+#ifdef HT_PIXEL_CLASSES
+// Program option HT_PIXEL_CLASSES (with HT_PIXEL_QUEUE and HT_PIXEL_SPREAD =
+// 1): before each render_passes launch the runtime launches
+// pixel_classes(w, h, scene), which traces one probe camera ray per pixel
+// (from a private RNG stream: the pixel's seed is neither read nor changed)
+// and marks the pixels whose probe ray hits the scene. The pixel queue then
+// hands out the marked pixels first and the others after them, each in the
+// golden-ratio order: a pixel whose camera rays leave for the sky costs one
+// scan per sample, the others several, so the long chains start first and
+// the cheap pixels fill the frame's end (longest processing time first).
+// Frames of more than HT_CLASS_CAP pixels get the golden-ratio order alone.
+// Pixels stay independent: the same canvas and seeds.
+#if !(defined(HT_PIXEL_QUEUE) && defined(HT_PIXEL_SPREAD) && HT_PIXEL_SPREAD == 1)
+#error "HT_PIXEL_CLASSES needs HT_PIXEL_QUEUE and HT_PIXEL_SPREAD=1"
+#endif
+#define HT_CLASS_CAP (1u << 23)
+__device__ uint ht_pixel_class[HT_CLASS_CAP / 32];  // bit p: pixel p's probe ray hit the scene
+
+template <class S> HD void pixel_classes_impl(uint w, uint h, const u8 *__restrict__ scene) {
+    typedef typename S::Object T;
+    TopScan<T>::prepare(S::object(scene));
+    __syncthreads();
+    const uint n = w * h;
+    const uint idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool have = idx < n;
+    Context ctx;
+    ctx.rng.state = idx * 0x9E3779B9u + 0x7F4A7C15u;  // private probe stream
+    ctx.hasher = hash_init();
+    Light<typename S::Geo> light;
+    color3 emission;
+    start_path<S>(scene, &ctx, pixel_setup(have ? idx % w : 0u, have ? idx / w : 0u, w, h), &light, &emission);
+    ctx.hasher = hash_init();
+    typename T::Cache cache;
+    // every lane of the wave takes part in the scan; lanes past the frame
+    // have no ray
+    real dist = TopScan<T>::detect(S::object(scene), &ctx, &cache, &light, have);
+    const unsigned long long m = __ballot(have && dist > (real)-0.5f);
+    if (__lane_id() == 0 && idx < n) {  // the wave's 64 pixels start at idx, a multiple of 64
+        ht_pixel_class[idx / 32u] = (uint)m;
+        if (idx + 32u < n) ht_pixel_class[idx / 32u + 1u] = (uint)(m >> 32);
+    }
+}
+HD bool pixel_class(uint p) { return (ht_pixel_class[p >> 5] >> (p & 31u)) & 1u; }
+#endif
+
 #ifdef HT_PIXEL_QUEUE
 // Program option HT_PIXEL_QUEUE: render_passes as a persistent kernel (the
 // runtime launches only the blocks that fit on the device at once, after
@@ -415,7 +460,13 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
     const uint n = w * h;
 #ifdef HT_PIXEL_SPREAD
     const PixelOrder order(n);
+#ifdef HT_PIXEL_CLASSES
+    // two rounds over the order: the marked pixels, then the others
+    const bool classes = n <= HT_CLASS_CAP;
+    const uint extent = classes ? 2u * order.extent : order.extent;
+#else
     const uint extent = order.extent;  // the counter runs over the order's items
+#endif
 #else
     const uint extent = n;
 #endif
@@ -455,7 +506,14 @@ HD void render_passes_queued(uint w, uint h, const u8 *__restrict__ scene, float
                 base = __shfl(base, (int)leader);
                 if (base + (uint)__popcll(m) >= extent) drained = true;  // the counter only grows
                 uint cand = base + (uint)__popcll(m & below);
-#ifdef HT_PIXEL_SPREAD
+#if defined(HT_PIXEL_CLASSES)
+                if (need && cand < extent) {
+                    const bool second = cand >= order.extent;
+                    const uint q = order.pixel(second ? cand - order.extent : cand);
+                    // a pixel of the other round: no pixel for this lane now (it asks again)
+                    cand = (classes && pixel_class(q) == second) ? n : q;
+                }
+#elif defined(HT_PIXEL_SPREAD)
                 if (need && cand < extent) cand = order.pixel(cand);
 #endif
                 if (need && cand < n) p.take(cand, w, h, scene, canvas, seeds, passes, S::HOPS);
@@ -531,9 +589,19 @@ HD void render_passes_impl(uint w, uint h, const u8 *__restrict__ scene, float4 
 #else
 #define HT_DEFINE_QUEUE_KERNEL
 #endif
+#ifdef HT_PIXEL_CLASSES
+#define HT_DEFINE_CLASS_KERNEL(S)                                                                     \
+    extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU)                                \
+        pixel_classes(unsigned w, unsigned h, const unsigned char *__restrict__ scene) {              \
+        ht::pixel_classes_impl<S>(w, h, scene);                                                       \
+    }
+#else
+#define HT_DEFINE_CLASS_KERNEL(S)
+#endif
 #define HT_DEFINE_RENDER_KERNELS(S)                                                                   \
     HT_DEFINE_BOUNDS_KERNEL(S)                                                                        \
     HT_DEFINE_QUEUE_KERNEL                                                                            \
+    HT_DEFINE_CLASS_KERNEL(S)                                                                         \
     extern "C" __global__ void __launch_bounds__(256, HT_WAVES_PER_EU) render(                        \
         unsigned w, unsigned h, const unsigned char *__restrict__ scene, float4 *__restrict__ canvas, \
         unsigned *__restrict__ seeds) {                                                               \

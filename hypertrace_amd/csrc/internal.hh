@@ -52,6 +52,9 @@ struct ht_prog {
     // pixel-queue programs: resets the pixel counter before each
     // render_passes launch, which then runs as many blocks as fit at once
     hipFunction_t pixel_queue_reset = nullptr;
+    // programs built with -DHT_PIXEL_CLASSES: the probe that orders the pixel
+    // queue, launched before each render_passes launch
+    hipFunction_t pixel_classes = nullptr;
     uint32_t resident_blocks = 0;
     // Wavefront form (programs whose root source adds the wavefront kernels):
     // kernels, queue sizes {extend-record words, shade-record words, keys,

@@ -486,6 +486,7 @@ extern "C" int ht_prog_build_ex(ht_ctx *ctx, const char *root_name, const char *
     p->render_batch = find("render_batch");
     p->bounds_fill = find("bounds_fill");
     p->pixel_queue_reset = find("pixel_queue_reset");
+    p->pixel_classes = find("pixel_classes");
     if (p->pixel_queue_reset && p->render_passes) {
         int per_cu = 0, cus = 0;
         e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p->render_passes, 256, 0);
@@ -681,6 +682,14 @@ int launch_render(ht_prog *prog, hipFunction_t fn, uint32_t w, uint32_t h, ht_bu
     void *sp = scene->ptr, *cp = canvas->ptr, *pp = seeds->ptr;
     std::lock_guard<std::mutex> lock(prog->launch_mu);
     if ((rc = ht_internal::fill_bounds(prog, sp))) return rc;
+    if (with_passes && prog->pixel_classes && n <= (1u << 23)) {
+        // the probe that marks the pixels the queue hands out first (the
+        // kernel's HT_CLASS_CAP: larger frames are not probed and the
+        // render kernel does not read the marks)
+        void *args_probe[] = {&w, &h, &sp};
+        HIP_TRY(hipModuleLaunchKernel(prog->pixel_classes, grid, 1, 1, block, 1, 1, 0, ctx->stream, args_probe,
+                                      nullptr));
+    }
     if (with_passes && prog->pixel_queue_reset) {
         // persistent: the blocks that fit at once pull pixels from the counter
         HIP_TRY(hipModuleLaunchKernel(prog->pixel_queue_reset, 1, 1, 1, 64, 1, 1, 0, ctx->stream, nullptr, nullptr));

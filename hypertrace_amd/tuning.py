@@ -40,7 +40,10 @@ pixels over the frame (1: the golden-ratio order t * a mod n; B = 2..64:
 one run of 64 / B pixels from each of B bands), so that a cluster of
 expensive pixels is spread over many waves whose other lanes finish early
 and help (with the pixel queue and the golden-ratio order the 640x480 cell
-frame takes 16.8 instead of 20.6 ms). Every
+frame takes 16.8 instead of 20.6 ms); HT_PIXEL_CLASSES (with the queue and
+the golden-ratio order) probes one camera ray per pixel before the frame
+and hands out the pixels whose probe hits the scene first, the sky pixels
+after them (3 % on the 640x480 cell frame). Every
 setting renders bit-identical images (only the schedule differs), so the
 tuner simply times one frame of the caller's workload per candidate on the
 GPU and keeps the fastest. Small frames (few waves per SIMD) prefer fewer
@@ -75,12 +78,15 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 4, True, True, True, True, True), (4, 4, True, 4, True, True, True, True, True, 16),
     (4, 4, True, 4, True, True, True, False, True), (4, 4, True, 4, True, True, True, False, True, 16),
     (4, 4, True, 4, True, True, True, True, True, 32, 1), (4, 4, True, 4, True, True, True, True, True, 16, 1),
-    (4, 3, True, 4, True, True, True, True, True, 32, 1), (4, 4, True, 4, True, True, True, True, True, 32, 64))
+    (4, 3, True, 4, True, True, True, True, True, 32, 1), (4, 4, True, 4, True, True, True, True, True, 32, 64),
+    (4, 4, True, 4, True, True, True, True, True, 32, 1, True), (4, 4, True, 4, True, True, True, True, True, 16, 1, True),
+    (4, 3, True, 4, True, True, True, True, True, 32, 1, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
             pixel_queue: bool = False, wave_prio: bool = False, group_queue: bool = False,
-            hit_order: bool = False, helpers: bool = False, sparse: int = 0, spread: int = 0) -> str:
+            hit_order: bool = False, helpers: bool = False, sparse: int = 0, spread: int = 0,
+            classes: bool = False) -> str:
     opts = f"{base} -DHT_CULL_UNROLL={int(unroll)} -DHT_WAVES_PER_EU={int(waves)}"
     if scalar_bounds:
         opts += " -DHT_SCALAR_BOUNDS=1"
@@ -100,10 +106,12 @@ def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool 
         opts += f" -DHT_SPARSE_L1={int(sparse)}"
     if spread:
         opts += f" -DHT_PIXEL_SPREAD={int(spread)}"
+    if classes:
+        opts += " -DHT_PIXEL_CLASSES=1"
     return opts
 
 
-def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool, bool, int, int]:
+def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, bool, bool, int, int, bool]:
     unroll, waves = int(candidate[0]), int(candidate[1])
     sb = bool(candidate[2]) if len(candidate) > 2 else False
     groups = int(candidate[3]) if len(candidate) > 3 else 0
@@ -114,7 +122,8 @@ def _knobs(candidate: Sequence) -> Tuple[int, int, bool, int, bool, bool, bool, 
     hl = bool(candidate[8]) if len(candidate) > 8 else False
     sparse = int(candidate[9]) if len(candidate) > 9 else 0
     spread = int(candidate[10]) if len(candidate) > 10 else 0
-    return unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse, spread
+    classes = bool(candidate[11]) if len(candidate) > 11 else False
+    return unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse, spread, classes
 
 
 def candidate_options(candidate: Sequence, base: str = STRICT_FP) -> str:
@@ -160,8 +169,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
     buf = EntityBuffer(ctx, scene)
     table = []
     for cand in candidates:
-        unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse, spread = _knobs(cand)
-        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho, hl, sparse, spread)
+        unroll, waves, sb, groups, queue, prio, gq, ho, hl, sparse, spread, classes = _knobs(cand)
+        opts = options(unroll, waves, base, sb, groups, queue, prio, gq, ho, hl, sparse, spread, classes)
         r = Render(ctx, scene, fp_options=opts)
         c = Canvas(ctx, (width, height), seed_mode, seed)
         r.render_passes(buf, c, 1)
@@ -174,7 +183,8 @@ def tune_render(ctx: Context, scene, width: int, height: int, spp: int, seed_mod
         res = r.program.kernel_info("render_passes")
         table.append({"unroll": unroll, "waves_per_eu": waves, "scalar_bounds": sb, "groups": groups,
                       "pixel_queue": queue, "wave_prio": prio, "group_queue": gq, "hit_order": ho, "helpers": hl,
-                      "sparse": sparse, "spread": spread, "ms": sorted(times)[len(times) // 2], "options": opts,
+                      "sparse": sparse, "spread": spread, "classes": classes, "ms": sorted(times)[len(times) // 2],
+                      "options": opts,
                       "vgprs": res["vgprs"], "scratch": res["scratch"]})
     best = pick(table, spill_tolerance)
     result = {"best": best, "table": table, "fastest_any": min(table, key=lambda t: t["ms"]),

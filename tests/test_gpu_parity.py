@@ -270,6 +270,12 @@ CULLED_VARIANTS = {
                            " -DHT_SPARSE_L1=32 -DHT_PIXEL_SPREAD=1",
     "q_hl_sp1": STRICT_FP + " -DHT_PIXEL_QUEUE=1 -DHT_HELPERS=1 -DHT_PIXEL_SPREAD=1",
     "sp1_w3": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_PIXEL_SPREAD=1 -DHT_WAVES_PER_EU=3",
+    # the queue hands out the pixels whose probe ray hits the scene first
+    # (HT_PIXEL_CLASSES, with the golden-ratio order)
+    "gq4_ho_q_hl_s32_sp1_pc": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1"
+                              " -DHT_PIXEL_QUEUE=1 -DHT_WAVE_PRIO=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1"
+                              " -DHT_SPARSE_L1=32 -DHT_PIXEL_SPREAD=1 -DHT_PIXEL_CLASSES=1",
+    "q_sp1_pc": STRICT_FP + " -DHT_PIXEL_QUEUE=1 -DHT_PIXEL_SPREAD=1 -DHT_PIXEL_CLASSES=1",
     "gq4_ho_q_hl_s32_sp64": STRICT_FP + " -DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_GROUP_QUEUE=1"
                             " -DHT_PIXEL_QUEUE=1 -DHT_WAVE_PRIO=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1"
                             " -DHT_SPARSE_L1=32 -DHT_PIXEL_SPREAD=64",
@@ -313,7 +319,9 @@ CULLED_VARIANTS = {
     ("hy_cells", True, "gq4_ho_q_hl_s32_sp64"), ("hy_cells", False, "gq4_ho_hl_s32_sp8_w3"),
     ("sdf_1024", False, "gq4_ho_q_hl_s32_sp64"), ("eu_ball", False, "q_hl_sp4"), ("hy_example", False, "sp64"),
     ("hy_cells", True, "gq4_ho_q_hl_s32_sp1"), ("sdf_1024", False, "gq4_ho_q_hl_s32_sp1"),
-    ("eu_example", False, "q_hl_sp1"), ("hy_cells", False, "sp1_w3")])
+    ("eu_example", False, "q_hl_sp1"), ("hy_cells", False, "sp1_w3"),
+    ("hy_cells", True, "gq4_ho_q_hl_s32_sp1_pc"), ("sdf_1024", False, "gq4_ho_q_hl_s32_sp1_pc"),
+    ("hy_example", False, "q_sp1_pc"), ("eu_ball", False, "q_sp1_pc")])
 def test_culled_scan_identical_to_plain_scan(ctx, name, full, variant):
     """The culled, compacted nearest-hit scan (flat over LDS or scalar-cache
     bounds, or walking the sphere tree with -DHT_CULL_TREE) must give
@@ -358,7 +366,8 @@ def test_helpers_with_russian_roulette(ctx, name, helpers):
 
 
 @pytest.mark.parametrize("variant", ["gq4_ho_prio", "gq4_ho_w3", "gq4_ho_hl", "gq4_ho_hl_s32", "gq4_ho_q_s16",
-                                     "gq4_ho_q_hl_s16", "gq4_ho_q_hl_s32_sp64", "gq4_ho_q_hl_s32_sp1"])
+                                     "gq4_ho_q_hl_s16", "gq4_ho_q_hl_s32_sp64", "gq4_ho_q_hl_s32_sp1",
+                                     "gq4_ho_q_hl_s32_sp1_pc"])
 @pytest.mark.parametrize("frame", ["640x480", "1920x1080"])
 def test_hit_order_identical_on_full_frames(ctx, frame, variant):
     """The hit-order scan (each passing pair's crossing interval: pairs that
@@ -386,7 +395,7 @@ def test_hit_order_identical_on_full_frames(ctx, frame, variant):
 @pytest.mark.parametrize("variant", ["pixel_queue", "sb_q_prio", "gq4_q_prio_w6", "gq4_ho_q_prio", "gq4_ho_hl",
                                      "gq4_ho_q_s16", "gq4_ho_q_hl_s16", "q_hl", "gq4_ho_q_hl_s32_sp64", "sp64",
                                      "q_hl_sp4", "gq4_ho_hl_s32_sp8_w3", "gq4_ho_q_hl_s32_sp1", "q_hl_sp1",
-                                     "sp1_w3"])
+                                     "sp1_w3", "gq4_ho_q_hl_s32_sp1_pc", "q_sp1_pc"])
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (65, 1), (17, 19), (257, 5)])
 def test_pixel_queue_ragged_frames(ctx, size, variant):
     """The pixel-queue and spread-order programs on frames smaller than a

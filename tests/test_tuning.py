@@ -17,7 +17,7 @@ def test_candidates_distinct():
 
 
 def test_knob_defaults():
-    assert _knobs((4, 5)) == (4, 5, False, 0, False, False, False, False, False, 0, 0)
+    assert _knobs((4, 5)) == (4, 5, False, 0, False, False, False, False, False, 0, 0, False)
     assert candidate_options((4, 4, True, 4, False, True, True)).endswith(
         "-DHT_SCALAR_BOUNDS=1 -DHT_CULL_GROUPS=4 -DHT_WAVE_PRIO=1 -DHT_GROUP_QUEUE=1")
     assert options(4, 5) == candidate_options((4, 5, False, 0, False))
@@ -33,7 +33,8 @@ def test_knob_macros_are_read_by_the_device_sources():
     for c in CANDIDATES:
         names |= set(re.findall(r"-D(HT_[A-Z0-9_]+)=", candidate_options(c)))
     assert {"HT_CULL_UNROLL", "HT_WAVES_PER_EU", "HT_SCALAR_BOUNDS", "HT_CULL_GROUPS", "HT_PIXEL_QUEUE",
-            "HT_WAVE_PRIO", "HT_GROUP_QUEUE", "HT_HIT_ORDER", "HT_HELPERS", "HT_SPARSE_L1", "HT_PIXEL_SPREAD"} <= names
+            "HT_WAVE_PRIO", "HT_GROUP_QUEUE", "HT_HIT_ORDER", "HT_HELPERS", "HT_SPARSE_L1", "HT_PIXEL_SPREAD",
+            "HT_PIXEL_CLASSES"} <= names
     for name in names:
         assert re.search(rf"#\s*(ifdef|ifndef|if|elif)\b[^\n]*\b{name}\b", source), name
 
@@ -96,6 +97,9 @@ def test_bench_fixed_knobs_and_gpu_count_guard():
     assert k["sparse"] == 32 and k["options"] == candidate_options((4, 4, True, 4, False, True, True, True, True, 32))
     assert k["options"].endswith("-DHT_HELPERS=1 -DHT_SPARSE_L1=32") and bench.count_mode(k) == 2004
     assert bench.fixed_knobs("4,4,sb,g4,q,p,gq,ho,hl,s32,sp1")["best"]["options"].endswith("-DHT_PIXEL_SPREAD=1")
+    k = bench.fixed_knobs("4,4,sb,g4,q,p,gq,ho,hl,s32,sp1,pc")["best"]
+    assert k["classes"] and k["options"] == candidate_options((4, 4, True, 4, True, True, True, True, True, 32, 1, True))
+    assert k["options"].endswith("-DHT_PIXEL_SPREAD=1 -DHT_PIXEL_CLASSES=1")
     k = bench.fixed_knobs("4,4,sb,g4,q,p,gq,ho,hl,s32,sp64")["best"]
     assert (k["sparse"], k["spread"], k["pixel_queue"]) == (32, 64, True)
     assert k["options"] == candidate_options((4, 4, True, 4, True, True, True, True, True, 32, 64))
