@@ -48,18 +48,21 @@ FP32_VECTOR_PEAK = 157.3  # TFLOP/s, MI355X published FP32 vector (packed / dual
 FP32_NONPACKED_PEAK = 78.6  # TFLOP/s without packed FP32 (this project compiles packed FP32 off)
 # HBM bytes per render_passes launch on this workload for the tuner's picks
 # (scalar bounds, groups of 4 with the group queue, hit order, sparse level
-# 1 at 32 or 16 rays, the pixel queue in the golden-ratio order, helper
-# lanes, wave priority), from the FETCH_SIZE / WRITE_SIZE counter runs
-# scaled by the calibration copy (FETCH x 2.00, WRITE x 1.00;
-# profiles/render_passes_hyperbolic_cells_culled.md), against 12.3 MB
-# algorithmic: the excess is the programs' 92 B per lane of register spills
-# (24 MB of scratch over the resident lanes, more than the L2 holds).
-# `traffic` is null for a pick whose counters were not read.
+# 1 at 32 or 16 rays, the pixel queue in the golden-ratio order with or
+# without the probe's ordering, helper lanes, wave priority), from the
+# FETCH_SIZE / WRITE_SIZE counter runs scaled by the calibration copy
+# (FETCH x 2.00, WRITE x 1.00; profiles/render_passes_hyperbolic_cells_culled.md),
+# against 12.3 MB algorithmic: the excess is the programs' 92-96 B per lane
+# of register spills (24-25 MB of scratch over the resident lanes, more
+# than the L2 holds). `traffic` is null for a pick whose counters were not read.
 _PICK = ("-ffp-contract=off -DHT_STRICT_FP=1 -DHT_CULL_UNROLL=4 -DHT_WAVES_PER_EU=4 -DHT_SCALAR_BOUNDS=1 "
          "-DHT_CULL_GROUPS=4 -DHT_PIXEL_QUEUE=1 -DHT_WAVE_PRIO=1 -DHT_GROUP_QUEUE=1 -DHT_HIT_ORDER=1 -DHT_HELPERS=1 "
          "-DHT_SPARSE_L1={} -DHT_PIXEL_SPREAD=1")
+_PROBED = " -DHT_PIXEL_CLASSES=1"
 MEASURED_TRAFFIC_BYTES = {_PICK.format(32): (2 * 35282 + 581161) * 1024,
-                          _PICK.format(16): (2 * 35960 + 572511) * 1024}
+                          _PICK.format(16): (2 * 35960 + 572511) * 1024,
+                          _PICK.format(32) + _PROBED: (2 * 40926 + 652623) * 1024,
+                          _PICK.format(16) + _PROBED: (2 * 41423 + 645645) * 1024}
 NO_CULL = "-ffp-contract=off -DHT_STRICT_FP=1 -DHT_NO_CULL=1"
 ORACLE_SEED_XOR = 0x0BAD5EED
 
