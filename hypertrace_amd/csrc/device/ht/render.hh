@@ -81,19 +81,27 @@ HD void start_path(const u8 *scene, Context *ctx, const PixelSetup &px, Light<ty
 }
 
 template <class S>
+HD void render_pixel_passes(uint w, uint h, uint idx, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
+                            uint *__restrict__ seeds, uint passes, int hops);
+
+template <class S>
 HD void render_pass_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *__restrict__ canvas,
                          uint *__restrict__ seeds) {
-    typedef typename S::Geo G;
     TopScan<typename S::Object>::prepare(S::object(scene));
     __syncthreads();
     uint idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= w * h) return;
+#ifdef HT_HELPERS
+    // one sample of render_pixel_passes: the same additions in the same
+    // order, with the lanes whose path has ended helping the wave's scans
+    render_pixel_passes<S>(w, h, idx, scene, canvas, seeds, 1u, S::HOPS);
+#else
     uint x = idx % w, y = idx / w;
     PixelSetup px = pixel_setup(x, y, w, h);
     Context ctx;
     ctx.rng.state = seeds[idx];
     ctx.hasher = hash_init();
-    Light<G> light;
+    Light<typename S::Geo> light;
     color3 emission;
     start_path<S>(scene, &ctx, px, &light, &emission);
     for (int i = 0; i < S::HOPS; ++i) {
@@ -106,6 +114,7 @@ HD void render_pass_impl(uint w, uint h, const u8 *__restrict__ scene, float4 *_
     acc.w += 1.0f;
     canvas[idx] = acc;
     seeds[idx] = ctx.rng.state;
+#endif
 }
 
 // `passes` samples of pixel idx with in-thread path regeneration, each path
