@@ -80,7 +80,8 @@ CANDIDATES: Tuple[Tuple, ...] = (
     (4, 4, True, 4, True, True, True, True, True, 32, 1), (4, 4, True, 4, True, True, True, True, True, 16, 1),
     (4, 3, True, 4, True, True, True, True, True, 32, 1), (4, 4, True, 4, True, True, True, True, True, 32, 64),
     (4, 4, True, 4, True, True, True, True, True, 32, 1, True), (4, 4, True, 4, True, True, True, True, True, 16, 1, True),
-    (4, 3, True, 4, True, True, True, True, True, 32, 1, True))
+    (4, 3, True, 4, True, True, True, True, True, 32, 1, True),
+    (4, 4, True, 4, True, True, True, False, True, 16, 1), (4, 4, True, 4, True, True, True, False, True, 16, 1, True))
 
 
 def options(unroll: int, waves: int, base: str = STRICT_FP, scalar_bounds: bool = False, groups: int = 0,
