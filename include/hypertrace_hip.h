@@ -70,7 +70,16 @@ int ht_prog_build(ht_ctx *ctx, const char *root_name, const char *root_src, int 
  * lanes take pixels from a per-program counter; launches of one program
  * are ordered on its context's stream, so they never share it),
  * -DHT_WAVE_PRIO=1 (each wave's issue priority steps down as its pixels'
- * samples complete, so the waves with the most work left issue first); and
+ * samples complete, so the waves with the most work left issue first),
+ * -DHT_GROUP_QUEUE=1, -DHT_HIT_ORDER=1, -DHT_SPARSE_L1=r and -DHT_HELPERS=1
+ * (the two-level scan compacted at both levels, its exact tests ordered
+ * and skipped by crossing intervals, level 1 over (group, ray) pairs in
+ * waves with at most r rays, lanes without a path joining the wave's
+ * scans), -DHT_PIXEL_SPREAD=1 or 2..64 (the order in which render_passes
+ * hands pixels to lanes: golden-ratio or banded) and -DHT_PIXEL_CLASSES=1
+ * (with the pixel queue: ht_render_passes first launches the program's
+ * pixel_classes probe, one camera ray per pixel from a private RNG stream,
+ * and the queue hands out the pixels whose probe hits the scene first); and
  * methods that do not: -DHT_RR_DEPTH=d (Russian roulette),
  * -DHT_HY_DIST_FP64=1 (hyperbolic distance in FP64, cancellation-free),
  * -DHT_EPS32=<float> (FP32 programs: this hit window instead of the
